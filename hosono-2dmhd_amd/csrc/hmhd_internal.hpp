@@ -1,0 +1,92 @@
+// hmhd_internal.hpp -- data shared by the host side and the kernels.
+//
+// One solve ("instance") keeps on the device:
+//   U[3]    three SoA FP64 state buffers [8][ny][nx], interior cells only.
+//           Guard cells are never stored: every read maps a padded index to
+//           the interior cell the reference's guard fill would copy
+//           (periodic wrap or outflow clamp, src/boundaryConditions.f90).
+//   FR      raw face fluxes [2 dirs][8 vars][my][mx] on the padded grid;
+//           FR[0](i,j) is the x-face at i-1/2, FR[1](i,j) the y-face at
+//           j-1/2 (src/gridBlock.f90:120-131).
+//   FLB     the two FLICD-corrected components [2][my][mx]: By on x-faces,
+//           Bx on y-faces (src/FLICDdivB.f90:33-58).
+//   MOOD state on the padded grid, one byte per cell: scheme, rc
+//           (recompute), rcF (recomputeFLICD), inadm (inadmissible face
+//           states), low (detect verdict), rec (cells to re-reconstruct in
+//           the inner loop), faceRR[4] (radius each face was last
+//           reconstructed with).
+//   ctl     per-instance flags read back by the host between MOOD passes.
+#pragma once
+
+#include <stdint.h>
+
+namespace hmhd {
+
+enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
+enum : int { NCONS = 8 };
+// ctl words: flags the kernels raise, the MOOD-loop state the host steers
+// with k_ctl, and per-stage pass / inner-iteration counters
+enum : int {
+    CTL_ANY_INADM = 0,  // an inadmissible reconstructed state was found
+    CTL_ANY_LOWER = 1,  // detect lowered a cell
+    CTL_RUN = 2,        // this instance is still inside its MOOD loop
+    CTL_MODE = 3,       // MODE_NEWPASS: reconstruct the rc cells; MODE_INNER: the rec cells
+    CTL_PASSES = 4,     // [3] MOOD passes per stage
+    CTL_INNER = 7,      // [3] inner admissibility iterations per stage
+    CTL_N = 16
+};
+enum : int { MODE_NEWPASS = 0, MODE_INNER = 1 };
+enum : int { OP_BEGIN = 0, OP_NEXT = 1 };
+
+constexpr int kMaxR = 3;
+constexpr int kMaxGpSets = 8;
+// prediction-vector block of radius rr inside one set: [4 faces][nq][ns]
+constexpr int gp_ns(int rr) { return 1 + 2 * rr * (rr + 1); }
+constexpr int gp_nq(int rr) { return rr + 1; }
+constexpr int gp_pv_off(int rr) { return rr <= 1 ? 0 : gp_pv_off(rr - 1) + 4 * gp_nq(rr - 1) * gp_ns(rr - 1); }
+constexpr int kPvSet = gp_pv_off(kMaxR + 1);  // 596 doubles
+
+struct Inst {
+    int32_t nx, ny, ngc, mx, my, R, bc, hydro;
+    int32_t gpset, active, pad0, pad1;
+    double dx, dy, mindxdy, maxdxdy;
+    double gamma, small_p, small_rho, cfl;
+    double dtdx, dtdy;
+    double* U[3];
+    double* V0;  // uploaded primitives for the first CFL, [8][ny][nx]: d,vx,vy,vz,p,bx,by,bz
+    double* FR;
+    double* FLB;
+    uint8_t* scheme;
+    uint8_t* rc;
+    uint8_t* rcF;
+    uint8_t* inadm;
+    uint8_t* low;
+    uint8_t* rec;
+    uint8_t* faceRR;  // [4][my][mx]
+    int32_t* ctl;
+    unsigned long long* cflmax;  // [2] bit patterns of the max speeds
+};
+
+// one tile of one instance; the tile grid depends on the kernel family
+struct Item {
+    int32_t inst;
+    int16_t tx, ty;
+};
+
+// tile sizes
+constexpr int kFaceTX = 16, kFaceTY = 16;  // face kernel (LDS-staged)
+constexpr int kCellT = 16;                 // update / mask kernels
+
+struct LaunchSet {
+    const Inst* insts;
+    const Item* face;
+    int nface;
+    const Item* upd;
+    int nupd;
+    const Item* pad;
+    int npad;
+    const Item* cell;  // interior tiles for CFL
+    int ncell;
+};
+
+}  // namespace hmhd

@@ -1,0 +1,901 @@
+// hmhd_kernels.hip -- gfx950 kernels of one SSP-RK3 stage and the CFL
+// reduction.  A stage is a sequence of MOOD passes; each pass is
+//
+//   k_face     GP reconstruction of both Riemann states of every owned face
+//              from an LDS tile of U, admissibility (CAD/PAD) of the
+//              reconstructed states, HLLC at the Gauss points and the
+//              weighted sum -> FR  (src/reconstruct.f90:77-141,
+//              src/getFluxes.f90:26-88, src/riemannSolver.f90:23-47)
+//   k_update   FLICD corrections, the unsplit FV update, MOOD detection and
+//              the RK combination of the updated cell (src/FLICDdivB.f90,
+//              src/timeStep.f90:107-118, src/detect.f90:74-217,
+//              src/timeStep.f90:165,169)
+//   k_lower_*  the order-lowering bookkeeping of src/gridBlock.f90:194-235
+//              written as a gather, so no cell is written by two threads
+//   k_sync_*   periodic MOOD image sync (src/boundaryConditions.f90:189-254)
+//
+// The first pass of a stage runs k_face in FULL mode: every face at the top
+// radius, admissibility checked on the way.  Later passes (and the rare
+// first-pass redo when a reconstructed state was inadmissible) run the
+// inner admissibility loop with k_recon_check / k_lower_inner and then
+// k_face in FLUX mode, which recomputes only the faces the reference's
+// getFluxes_ would recompute, from the radius each side was last
+// reconstructed with.
+#include <hip/hip_runtime.h>
+
+#include "hmhd_internal.hpp"
+#include "hmhd_kernels.hpp"
+#include "hmhd_math.hpp"
+
+namespace hmhd {
+
+__constant__ double c_pv[kMaxGpSets][kPvSet];
+__constant__ double c_qw[kMaxGpSets][kMaxR + 1][4];
+
+// ------------------------------------------------------------------ //
+// stencil of radius RR in the reference's order: i outer, j inner,
+// |i|+|j| <= RR (src/GP.f90:85-95)
+template <int RR>
+struct Sten {
+    static constexpr int n = gp_ns(RR);
+    int x[n], y[n];
+    constexpr Sten() : x(), y() {
+        int c = 0;
+        for (int i = -RR; i <= RR; i++)
+            for (int j = -RR; j <= RR; j++)
+                if ((i < 0 ? -i : i) + (j < 0 ? -j : j) <= RR) {
+                    x[c] = i;
+                    y[c] = j;
+                    c++;
+                }
+    }
+};
+
+// padded index -> interior index along one axis (guard fill as index map)
+__device__ __forceinline__ int map_axis(int p, int ngc, int n, int bc) {
+    int k = p - ngc;
+    if (bc == BC_PERIODIC) {
+        k += (k < 0) ? n : 0;
+        k -= (k >= n) ? n : 0;
+    } else {
+        k = k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+    }
+    return k;
+}
+
+__device__ __forceinline__ Phys phys_of(const Inst& I) {
+    Phys ph;
+    ph.gamma = I.gamma;
+    ph.small_p = I.small_p;
+    ph.small_rho = I.small_rho;
+    ph.hydro = I.hydro;
+    return ph;
+}
+
+// GP point value at quadrature point q of face f of the cell whose stencil
+// the accessor A is centred on; radius RR (src/reconstruct.f90:109-124)
+template <int RR, class A>
+__device__ __forceinline__ void gp_point(const A& a, const double* __restrict__ w, double* out) {
+    constexpr Sten<RR> st{};
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) out[v] = 0.0;
+#pragma unroll
+    for (int k = 0; k < st.n; k++) {
+        const double wk = w[k];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) out[v] = out[v] + wk * a.at(v, st.x[k], st.y[k]);
+    }
+}
+
+template <int LX, int LP>
+struct LdsAcc {
+    const double* s;  // points at the centre cell of plane 0
+    __device__ __forceinline__ double at(int v, int dx, int dy) const { return s[v * LP + dy * LX + dx]; }
+};
+
+struct GlobAcc {
+    const double* U;
+    size_t plane;
+    int i, j, ngc, nx, ny, bc;
+    __device__ __forceinline__ double at(int v, int dx, int dy) const {
+        int kx = map_axis(i + dx, ngc, nx, bc), ky = map_axis(j + dy, ngc, ny, bc);
+        return U[v * plane + (size_t)ky * nx + kx];
+    }
+};
+
+__device__ __forceinline__ const double* pv_ptr(int set, int rr, int f, int q) {
+    return &c_pv[set][gp_pv_off(rr) + (f * gp_nq(rr) + q) * gp_ns(rr)];
+}
+
+// value of one Riemann state at point q from a side reconstructed with
+// radius rr (rr == 0: the cell average itself)
+template <int R, class A>
+__device__ __forceinline__ void side_value(const A& a, int set, int rr, int f, int q, double* out) {
+    if (rr == 0) {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) out[v] = a.at(v, 0, 0);
+    } else if (rr == R) {
+        gp_point<R>(a, pv_ptr(set, R, f, q), out);
+    } else if (R >= 3 && rr == 2) {
+        gp_point<(R >= 3 ? 2 : 1)>(a, pv_ptr(set, R >= 3 ? 2 : 1, f, q), out);
+    } else {
+        gp_point<1>(a, pv_ptr(set, 1, f, q), out);
+    }
+}
+
+enum : int { FACE_N = 0, FACE_S = 1, FACE_E = 2, FACE_W = 3 };
+
+// Compiler-only memory barrier: the stencil values of a face are re-read
+// from LDS for each Gauss point instead of being held in registers across
+// the point loop (2 x 13 x 8 doubles at R = 2 would not fit).
+__device__ __forceinline__ void reload_barrier() { asm volatile("" ::: "memory"); }
+
+// the rest of a MOOD pass (fluxes, update, detect, lowering, sync) runs
+// only for instances inside their MOOD loop whose inner admissibility loop
+// has converged
+__device__ __forceinline__ bool pass_ready(const Inst& I) {
+    return I.ctl[CTL_RUN] != 0 && I.ctl[CTL_ANY_INADM] == 0;
+}
+
+template <int DIR>
+__device__ __forceinline__ void accumulate_flux(double* acc, const Phys& ph, const double* uL, const double* uR,
+                                                double w, bool first, bool fog) {
+    double F[NCONS];
+    hllc<DIR>(ph, uL, uR, F);
+    if (fog) {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) acc[v] = F[v];
+    } else {
+        if (first) {
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) acc[v] = 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) acc[v] = acc[v] + w * F[v];
+    }
+}
+
+// ------------------------------------------------------------------ //
+// stage set-up: scheme = R, rc = rcF = true everywhere
+// (src/timeStep.f90:79-81); inadm cleared; every face at radius R
+__global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ insts, const Item* __restrict__ items) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i >= I.mx || j >= I.my) return;
+    const size_t idx = (size_t)j * I.mx + i, plane = (size_t)I.mx * I.my;
+    const uint8_t R = (uint8_t)I.R;
+    I.scheme[idx] = R;
+    I.rc[idx] = 1;
+    I.rcF[idx] = 1;
+    I.inadm[idx] = 0;
+#pragma unroll
+    for (int f = 0; f < 4; f++) I.faceRR[f * plane + idx] = R;
+}
+
+// ------------------------------------------------------------------ //
+// face kernel: each thread owns the x-face at i-1/2 and the y-face at j-1/2
+// of padded cell (i, j); the tile domain is [s-2, e+3]^2.
+template <int R, bool FULL>
+__global__ __launch_bounds__(kFaceTX* kFaceTY) void k_face(const Inst* __restrict__ insts,
+                                                           const Item* __restrict__ items, int in_buf) {
+    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1;
+    constexpr int LX = TX + 2 * R + 1, LY = TY + 2 * R + 1, LP = LX * LY;
+    __shared__ double sU[NCONS * LP];
+
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int i0 = s - 2 + it.tx * TX, j0 = s - 2 + it.ty * TY;
+    const int i = i0 + lx, j = j0 + ly;
+    const bool in_dom = (i <= ex + 3) && (j <= ey + 3);
+    const bool flux_cell = in_dom && i >= s - 1 && i <= ex + 2 && j >= s - 1 && j <= ey + 2;
+    const size_t idx = (size_t)j * mx + i;
+
+    bool do_x, do_y;
+    if (FULL) {
+        do_x = in_dom && j <= ey + 2;
+        do_y = in_dom && i <= ex + 2;
+    } else {
+        if (!pass_ready(I)) return;
+        do_x = flux_cell && (I.rc[idx] || I.rc[idx - 1]);
+        do_y = flux_cell && (I.rc[idx] || I.rc[idx - mx]);
+        if (!__syncthreads_or(do_x || do_y)) return;
+    }
+
+    // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
+    {
+        const double* __restrict__ U = I.U[in_buf];
+        const size_t plane = (size_t)I.nx * I.ny;
+        const int oi = i0 - H, oj = j0 - H;
+        for (int c = threadIdx.x; c < LP; c += TX * TY) {
+            const int cy = c / LX, cx = c - cy * LX;
+            const int kx = map_axis(oi + cx, s, I.nx, I.bc), ky = map_axis(oj + cy, s, I.ny, I.bc);
+            const double* src = U + (size_t)ky * I.nx + kx;
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) sU[v * LP + c] = src[v * plane];
+        }
+    }
+    __syncthreads();
+    if (!(do_x || do_y)) return;
+
+    const Phys ph = phys_of(I);
+    const int set = I.gpset;
+    const size_t fplane = (size_t)mx * I.my;
+    const int cc = (ly + H) * LX + (lx + H);  // LDS offset of cell (i, j)
+    const LdsAcc<LX, LP> aC{sU + cc}, aW{sU + cc - 1}, aS{sU + cc - LX};
+
+    if (FULL) {
+        constexpr int NQ = gp_nq(R);
+        // x-face: east state of (i-1, j) | west state of (i, j)
+        if (do_x) {
+            bool badL = false, badR = false;
+            double acc[NCONS];
+#pragma unroll 1
+            for (int q = 0; q < NQ; q++) {
+                double uL[NCONS], uR[NCONS];
+                reload_barrier();
+                gp_point<R>(aW, pv_ptr(set, R, FACE_E, q), uL);
+                gp_point<R>(aC, pv_ptr(set, R, FACE_W, q), uR);
+                badL |= !admissible(uL[DENS], cons2pres(ph, uL));
+                badR |= !admissible(uR[DENS], cons2pres(ph, uR));
+                if (flux_cell) accumulate_flux<0>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
+            }
+            if (badL && i >= s - 1) {
+                I.inadm[idx - 1] = 1;
+                I.ctl[CTL_ANY_INADM] = 1;
+            }
+            if (badR && i <= ex + 2) {
+                I.inadm[idx] = 1;
+                I.ctl[CTL_ANY_INADM] = 1;
+            }
+            if (flux_cell) {
+#pragma unroll
+                for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
+                I.FLB[idx] = acc[MAGY];
+            }
+        }
+        // y-face: north state of (i, j-1) | south state of (i, j)
+        if (do_y) {
+            bool badL = false, badR = false;
+            double acc[NCONS];
+#pragma unroll 1
+            for (int q = 0; q < NQ; q++) {
+                double uL[NCONS], uR[NCONS];
+                reload_barrier();
+                gp_point<R>(aS, pv_ptr(set, R, FACE_N, q), uL);
+                gp_point<R>(aC, pv_ptr(set, R, FACE_S, q), uR);
+                badL |= !admissible(uL[DENS], cons2pres(ph, uL));
+                badR |= !admissible(uR[DENS], cons2pres(ph, uR));
+                if (flux_cell) accumulate_flux<1>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
+            }
+            if (badL && j >= s - 1) {
+                I.inadm[idx - mx] = 1;
+                I.ctl[CTL_ANY_INADM] = 1;
+            }
+            if (badR && j <= ey + 2) {
+                I.inadm[idx] = 1;
+                I.ctl[CTL_ANY_INADM] = 1;
+            }
+            if (flux_cell) {
+#pragma unroll
+                for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
+                I.FLB[fplane + idx] = acc[MAGX];
+            }
+        }
+    } else {
+        if (do_x) {
+            const int rrL = I.faceRR[FACE_E * fplane + idx - 1], rrR = I.faceRR[FACE_W * fplane + idx];
+            const int sc = I.scheme[idx], sw = I.scheme[idx - 1];
+            const int rrF = sc < sw ? sc : sw;
+            const int nq = rrF == 0 ? 1 : gp_nq(rrF);
+            double acc[NCONS];
+            for (int q = 0; q < nq; q++) {
+                double uL[NCONS], uR[NCONS];
+                reload_barrier();
+                side_value<R>(aW, set, rrL, FACE_E, q, uL);
+                side_value<R>(aC, set, rrR, FACE_W, q, uR);
+                accumulate_flux<0>(acc, ph, uL, uR, c_qw[set][rrF][q], q == 0, rrF == 0);
+            }
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
+            I.FLB[idx] = acc[MAGY];
+        }
+        if (do_y) {
+            const int rrL = I.faceRR[FACE_N * fplane + idx - mx], rrR = I.faceRR[FACE_S * fplane + idx];
+            const int sc = I.scheme[idx], ss = I.scheme[idx - mx];
+            const int rrF = sc < ss ? sc : ss;
+            const int nq = rrF == 0 ? 1 : gp_nq(rrF);
+            double acc[NCONS];
+            for (int q = 0; q < nq; q++) {
+                double uL[NCONS], uR[NCONS];
+                reload_barrier();
+                side_value<R>(aS, set, rrL, FACE_N, q, uL);
+                side_value<R>(aC, set, rrR, FACE_S, q, uR);
+                accumulate_flux<1>(acc, ph, uL, uR, c_qw[set][rrF][q], q == 0, rrF == 0);
+            }
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
+            I.FLB[fplane + idx] = acc[MAGX];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ //
+// inner admissibility loop, reconstruction half: every flagged cell of
+// [s-2, e+2]^2 reconstructs its four faces at the current EDP1 radius,
+// records that radius per face, and flags itself if a state is
+// inadmissible (src/reconstruct.f90:77-141).
+template <int R>
+__global__ __launch_bounds__(256) void k_recon_check(const Inst* __restrict__ insts, const Item* __restrict__ items,
+                                                     int in_buf) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    if (!I.ctl[CTL_RUN]) return;
+    const bool use_rc = I.ctl[CTL_MODE] == MODE_NEWPASS;
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
+    if (i < s - 2 || i > ex + 2 || j < s - 2 || j > ey + 2) return;
+    const size_t idx = (size_t)j * mx + i, fplane = (size_t)mx * I.my;
+    const bool flag = use_rc ? I.rc[idx] : I.rec[idx];
+    if (!flag) {
+        I.inadm[idx] = 0;
+        return;
+    }
+    const Phys ph = phys_of(I);
+    const GlobAcc a{I.U[in_buf], (size_t)I.nx * I.ny, i, j, s, I.nx, I.ny, I.bc};
+    const int sc = I.scheme[idx];
+    const int nb[4] = {I.scheme[idx + mx], I.scheme[idx - mx], I.scheme[idx + 1], I.scheme[idx - 1]};
+    bool bad = false;
+    for (int f = 0; f < 4; f++) {
+        const int rr = sc < nb[f] ? sc : nb[f];
+        I.faceRR[f * fplane + idx] = (uint8_t)rr;
+        if (rr == 0) continue;
+        for (int q = 0; q < gp_nq(rr); q++) {
+            double u[NCONS];
+            reload_barrier();
+            side_value<R>(a, I.gpset, rr, f, q, u);
+            bad |= !admissible(u[DENS], cons2pres(ph, u));
+        }
+    }
+    I.inadm[idx] = bad ? 1 : 0;
+    if (bad) I.ctl[CTL_ANY_INADM] = 1;
+}
+
+__device__ __forceinline__ uint8_t gather_or(const uint8_t* a, int i, int j, int mx, int my, int rad) {
+    uint8_t r = 0;
+    for (int dj = -rad; dj <= rad; dj++) {
+        const int jj = j + dj;
+        if (jj < 0 || jj >= my) continue;
+        const int w = rad - (dj < 0 ? -dj : dj);
+        for (int di = -w; di <= w; di++) {
+            const int ii = i + di;
+            if (ii < 0 || ii >= mx) continue;
+            r |= a[(size_t)jj * mx + ii];
+        }
+    }
+    return r;
+}
+
+// lowering of every inadmissible cell, as a gather (src/reconstruct.f90:146-171
+// with src/gridBlock.f90:194-235): rc |= cross, rcF |= diamond, rec = cross
+__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const Item* __restrict__ items) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    if (!I.ctl[CTL_RUN] || I.ctl[CTL_MODE] != MODE_INNER) return;
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i >= I.mx || j >= I.my) return;
+    const size_t idx = (size_t)j * I.mx + i;
+    const uint8_t cross = gather_or(I.inadm, i, j, I.mx, I.my, 1);
+    const uint8_t diam = gather_or(I.inadm, i, j, I.mx, I.my, 2);
+    if (I.inadm[idx]) {
+        const int sc = I.scheme[idx];
+        I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
+    }
+    I.rc[idx] |= cross;
+    I.rcF[idx] |= diam;
+    I.rec[idx] = cross;
+}
+
+// after detect: rc = cross(low), rcF = diamond(low), scheme lowered
+// (src/detect.f90:60-72,219-234)
+__global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const Item* __restrict__ items) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    if (!pass_ready(I)) return;
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i >= I.mx || j >= I.my) return;
+    const size_t idx = (size_t)j * I.mx + i;
+    I.rc[idx] = gather_or(I.low, i, j, I.mx, I.my, 1);
+    I.rcF[idx] = gather_or(I.low, i, j, I.mx, I.my, 2);
+    if (I.low[idx]) {
+        const int sc = I.scheme[idx];
+        I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
+    }
+}
+
+// periodic MOOD sync: x sweep over every row, then y sweep over every column
+// (src/boundaryConditions.f90:205-252); needs N >= 2 NGC so pairs are disjoint
+__device__ __forceinline__ void sync_pair(const Inst& I, size_t a, size_t b) {
+    uint8_t e = I.rcF[a] | I.rcF[b];
+    I.rcF[a] = e;
+    I.rcF[b] = e;
+    e = I.rc[a] | I.rc[b];
+    I.rc[a] = e;
+    I.rc[b] = e;
+    const uint8_t sa = I.scheme[a], sb = I.scheme[b];
+    const uint8_t m = sa < sb ? sa : sb;
+    I.scheme[a] = m;
+    I.scheme[b] = m;
+}
+
+__global__ __launch_bounds__(256) void k_sync_x(const Inst* __restrict__ insts, const Item* __restrict__ items) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    if (I.bc != BC_PERIODIC || !pass_ready(I)) return;
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i >= I.mx || j >= I.my) return;
+    const int s = I.ngc, e = I.ngc + I.nx - 1;
+    if (i >= s && i <= e) return;
+    const int ip = i < s ? i + I.nx : i - I.nx;
+    sync_pair(I, (size_t)j * I.mx + i, (size_t)j * I.mx + ip);
+}
+
+__global__ __launch_bounds__(256) void k_sync_y(const Inst* __restrict__ insts, const Item* __restrict__ items) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    if (I.bc != BC_PERIODIC || !pass_ready(I)) return;
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i >= I.mx || j >= I.my) return;
+    const int s = I.ngc, e = I.ngc + I.ny - 1;
+    if (j >= s && j <= e) return;
+    const int jp = j < s ? j + I.ny : j - I.ny;
+    sync_pair(I, (size_t)j * I.mx + i, (size_t)jp * I.mx + i);
+}
+
+// ------------------------------------------------------------------ //
+// FLICD + FV update + detect + RK combination, cells [s, e+1]^2.
+// stage 0: out = Unew; stage 1: out = (3 base + Unew)/4;
+// stage 2: out = (base + 2 Unew)/3.
+
+struct FaceRead {
+    const double* FR;
+    size_t fp;
+    int mx;
+    __device__ __forceinline__ double x(int v, int i, int j) const { return FR[v * fp + (size_t)j * mx + i]; }
+    __device__ __forceinline__ double y(int v, int i, int j) const {
+        return FR[(NCONS + v) * fp + (size_t)j * mx + i];
+    }
+    // src/FLICDdivB.f90:38-45
+    __device__ __forceinline__ double corr_x(int i, int j) const {
+        return (2.0 * x(MAGY, i, j) + x(MAGY, i + 1, j) + x(MAGY, i - 1, j) - y(MAGX, i, j + 1) - y(MAGX, i, j) -
+                y(MAGX, i - 1, j) - y(MAGX, i - 1, j + 1)) /
+               8.0;
+    }
+    // src/FLICDdivB.f90:47-54
+    __device__ __forceinline__ double corr_y(int i, int j) const {
+        return (2.0 * y(MAGX, i, j) + y(MAGX, i, j + 1) + y(MAGX, i, j - 1) - x(MAGY, i + 1, j) - x(MAGY, i, j) -
+                x(MAGY, i, j - 1) - x(MAGY, i + 1, j - 1)) /
+               8.0;
+    }
+};
+
+// NaN-skipping MINVAL / MAXVAL of five values
+__device__ __forceinline__ double minval5(const double* x) {
+    double m = x[0];
+#pragma unroll
+    for (int k = 1; k < 5; k++)
+        if (__builtin_isnan(m) || x[k] < m) m = x[k];
+    return m;
+}
+__device__ __forceinline__ double maxval5(const double* x) {
+    double m = x[0];
+#pragma unroll
+    for (int k = 1; k < 5; k++)
+        if (__builtin_isnan(m) || x[k] > m) m = x[k];
+    return m;
+}
+
+struct CellRead {
+    const double* U;
+    size_t plane;
+    int ngc, nx, ny, bc;
+    __device__ __forceinline__ size_t off(int i, int j) const {
+        return (size_t)map_axis(j, ngc, ny, bc) * nx + map_axis(i, ngc, nx, bc);
+    }
+    __device__ __forceinline__ void load(int i, int j, double* u) const {
+        const size_t o = off(i, j);
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) u[v] = U[v * plane + o];
+    }
+    __device__ __forceinline__ double dens(int i, int j) const { return U[off(i, j)]; }
+};
+
+// src/detect.f90:83-213 for one checked cell; true = lower the order
+__device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const CellRead& cr, int i, int j,
+                                            const double* un, const double* uc) {
+    const double presNew = cons2pres(ph, un);
+    if (!admissible(un[DENS], presNew)) return true;
+    double u[NCONS];
+    const Prim V = cons2prim(ph, uc);
+    cr.load(i + 1, j, u);
+    const Prim Vip = cons2prim(ph, u);
+    cr.load(i - 1, j, u);
+    const Prim Vim = cons2prim(ph, u);
+    cr.load(i, j + 1, u);
+    const Prim Vjp = cons2prim(ph, u);
+    cr.load(i, j - 1, u);
+    const Prim Vjm = cons2prim(ph, u);
+    double div_vel = (Vip.vx - Vim.vx) / (2.0 * I.dx);
+    div_vel = div_vel + (Vjp.vy - Vjm.vy) / (2.0 * I.dy);
+    const bool shock1 = div_vel < -(1.0 * (I.maxdxdy * I.maxdxdy));
+    double mach = V.vx * V.vx + V.vy * V.vy;
+    mach = V.d * mach / (ph.gamma * V.p);
+    mach = __builtin_sqrt(mach);
+    const bool shock2 = mach > 0.2;
+    double pg = __builtin_fabs(Vip.p - Vim.p) / fminF(Vip.p, Vim.p);
+    pg = pg + __builtin_fabs(Vjp.p - Vjm.p) / fminF(Vjp.p, Vjm.p);
+    const bool shock3 = pg > 10.0 * I.maxdxdy;
+    if (!(shock1 && shock2 && shock3)) return false;
+
+    const double densNew = un[DENS];
+    const double d1 = uc[DENS], d2 = cr.dens(i + 1, j), d3 = cr.dens(i - 1, j), d4 = cr.dens(i, j + 1),
+                 d5 = cr.dens(i, j - 1);
+    const double mn = fminF(fminF(fminF(fminF(d1, d2), d3), d4), d5);
+    const double mxd = fmaxF(fmaxF(fmaxF(fmaxF(d1, d2), d3), d4), d5);
+    if (mxd - mn < I.mindxdy * I.mindxdy * I.mindxdy) return false;
+    if ((mn <= densNew) && (densNew <= mxd)) return false;
+
+    // u2 smoothness on the five cells of the cross (src/detect.f90:177-213)
+    double ddx[5], ddy[5];
+    const double dx2 = I.dx * I.dx, dy2 = I.dy * I.dy;
+    const int ci[5] = {-1, 0, 0, 0, 1}, cj[5] = {0, -1, 0, 1, 0};
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        const int ii = i + ci[c], jj = j + cj[c];
+        const double dcc = cr.dens(ii, jj);
+        ddx[c] = (cr.dens(ii + 1, jj) - 2.0 * dcc + cr.dens(ii - 1, jj)) / dx2;
+        ddy[c] = (cr.dens(ii, jj + 1) - 2.0 * dcc + cr.dens(ii, jj - 1)) / dy2;
+    }
+    const double mnx = minval5(ddx), mny = minval5(ddy), mxx = maxval5(ddx), mxy = maxval5(ddy);
+    const double m = I.mindxdy;
+    const bool sx1 = mnx * mxx > -m, sy1 = mny * mxy > -m;
+    const double ax = fmaxF(__builtin_fabs(mnx), __builtin_fabs(mxx)), ay = fmaxF(__builtin_fabs(mny), __builtin_fabs(mxy));
+    const bool sx2 = ax < m, sy2 = ay < m;
+    const bool sx3 = fminF(__builtin_fabs(mnx), __builtin_fabs(mxx)) >= 0.5 * ax;
+    const bool sy3 = fminF(__builtin_fabs(mny), __builtin_fabs(mxy)) >= 0.5 * ay;
+    const bool smooth1 = sx1 && sx2 && sy1 && sy2;
+    const bool smooth2 = sx1 && sx3 && sy1 && sy3;
+    return !(smooth1 || smooth2);
+}
+
+__global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Item* __restrict__ items,
+                                                int in_buf, int out_buf, int base_buf, int stage) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    if (!pass_ready(I)) return;
+    const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
+    const int i = s + it.tx * kCellT + (threadIdx.x % kCellT), j = s + it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i > ex + 1 || j > ey + 1) return;
+    const size_t idx = (size_t)j * mx + i, fp = (size_t)mx * I.my;
+    const FaceRead fr{I.FR, fp, mx};
+    const bool own = I.rcF[idx];
+    double cx = 0.0, cy = 0.0;
+    if (own) {
+        cx = fr.corr_x(i, j);
+        cy = fr.corr_y(i, j);
+        I.FLB[idx] = cx;
+        I.FLB[fp + idx] = cy;
+    }
+    if (i > ex || j > ey) return;
+    const size_t ci = (size_t)(j - s) * I.nx + (i - s), plane = (size_t)I.nx * I.ny;
+    if (!own) {
+        I.low[idx] = 0;
+        return;
+    }
+    const double cxe = I.rcF[idx + 1] ? fr.corr_x(i + 1, j) : I.FLB[idx + 1];
+    const double cyn = I.rcF[idx + mx] ? fr.corr_y(i, j + 1) : I.FLB[fp + idx + mx];
+    const double dtdx = I.dtdx, dtdy = I.dtdy;
+    const double* __restrict__ Uin = I.U[in_buf];
+    double uc[NCONS], un[NCONS];
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) {
+        double fxm = fr.x(v, i, j), fxp = fr.x(v, i + 1, j), fym = fr.y(v, i, j), fyp = fr.y(v, i, j + 1);
+        if (v == MAGX) {
+            fxm = 0.0;
+            fxp = 0.0;
+            fym = cy;
+            fyp = cyn;
+        } else if (v == MAGY) {
+            fxm = cx;
+            fxp = cxe;
+            fym = 0.0;
+            fyp = 0.0;
+        }
+        uc[v] = Uin[v * plane + ci];
+        un[v] = uc[v] - dtdx * (fxp - fxm) - dtdy * (fyp - fym);
+    }
+    bool lower = false;
+    if (I.scheme[idx] > 0) {
+        const Phys ph = phys_of(I);
+        const CellRead cr{Uin, plane, s, I.nx, I.ny, I.bc};
+        lower = detect_cell(I, ph, cr, i, j, un, uc);
+    }
+    I.low[idx] = lower ? 1 : 0;
+    if (lower) I.ctl[CTL_ANY_LOWER] = 1;
+    double* __restrict__ Uout = I.U[out_buf];
+    if (stage == 0) {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = un[v];
+    } else if (stage == 1) {
+        const double* __restrict__ Ub = I.U[base_buf];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (3.0 * Ub[v * plane + ci] + un[v]) / 4.0;
+    } else {
+        const double* __restrict__ Ub = I.U[base_buf];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (Ub[v * plane + ci] + 2.0 * un[v]) / 3.0;
+    }
+}
+
+// MOOD-loop control between passes, one thread per instance.
+// OP_BEGIN starts a stage; OP_NEXT decides, from the flags of the pass just
+// run, whether the instance continues its inner admissibility loop
+// (MODE_INNER), starts a new MOOD pass (MODE_NEWPASS) or leaves the loop.
+__global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int stage) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ninst) return;
+    int32_t* c = insts[k].ctl;
+    if (op == OP_BEGIN) {
+        c[CTL_ANY_INADM] = 0;
+        c[CTL_ANY_LOWER] = 0;
+        c[CTL_RUN] = 1;
+        c[CTL_MODE] = MODE_NEWPASS;
+        c[CTL_PASSES + stage] = 1;
+        c[CTL_INNER + stage] = 1;
+    } else if (c[CTL_RUN]) {
+        if (c[CTL_ANY_INADM]) {
+            c[CTL_ANY_INADM] = 0;
+            c[CTL_MODE] = MODE_INNER;
+            c[CTL_INNER + stage] += 1;
+        } else if (c[CTL_ANY_LOWER]) {
+            c[CTL_ANY_LOWER] = 0;
+            c[CTL_MODE] = MODE_NEWPASS;
+            c[CTL_PASSES + stage] += 1;
+            c[CTL_INNER + stage] += 1;
+        } else {
+            c[CTL_RUN] = 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ //
+// CFL: max over the interior of |v| + c_f from V = cons2prim(U), or from the
+// uploaded primitives before the first step (src/cfl.f90:28-48)
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, const Item* __restrict__ items,
+                                             int buf, int use_v0) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    const int ki = it.tx * kCellT + (threadIdx.x % kCellT), kj = it.ty * kCellT + (threadIdx.x / kCellT);
+    unsigned long long bx = 0, by = 0;
+    if (ki < I.nx && kj < I.ny) {
+        const size_t plane = (size_t)I.nx * I.ny, o = (size_t)kj * I.nx + ki;
+        double d, vx, vy, p, b1, b2, b3;
+        if (use_v0) {
+            const double* V = I.V0;
+            d = V[o];
+            vx = V[plane + o];
+            vy = V[2 * plane + o];
+            p = V[4 * plane + o];
+            b1 = V[5 * plane + o];
+            b2 = V[6 * plane + o];
+            b3 = V[7 * plane + o];
+        } else {
+            const Phys ph = phys_of(I);
+            double u[NCONS];
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) u[v] = I.U[buf][v * plane + o];
+            const Prim V = cons2prim(ph, u);
+            d = V.d;
+            vx = V.vx;
+            vy = V.vy;
+            p = V.p;
+            b1 = V.bx;
+            b2 = V.by;
+            b3 = V.bz;
+        }
+        double cs;
+        if (I.hydro) {
+            cs = __builtin_sqrt(I.gamma * p / d);
+        } else {
+            const double BB = b1 * b1 + b2 * b2 + b3 * b3;
+            double ab[3] = {__builtin_fabs(b1), __builtin_fabs(b2), __builtin_fabs(b3)};
+            double bn = ab[0];
+            for (int k = 1; k < 3; k++)
+                if (__builtin_isnan(bn) || ab[k] < bn) bn = ab[k];
+            cs = eigen_cfs(I.gamma, d, p, BB, bn);
+        }
+        const double xl = __builtin_fabs(vx) + cs, yl = __builtin_fabs(vy) + cs;
+        bx = __double_as_longlong(xl);
+        by = __double_as_longlong(yl);
+        // negative zero cannot occur (|v| + c >= 0); NaN sorts above +inf
+    }
+    bx = wave_max_u64(bx);
+    by = wave_max_u64(by);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&I.cflmax[0], bx);
+        atomicMax(&I.cflmax[1], by);
+    }
+}
+
+__global__ void k_cfl_reset(const Inst* __restrict__ insts, int ninst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ninst) {
+        insts[k].cflmax[0] = 0ull;
+        insts[k].cflmax[1] = 0ull;
+    }
+}
+
+// dt/dx and dt/dy of each instance from a device dt array
+__global__ void k_set_dt(Inst* __restrict__ insts, const double* __restrict__ dt, int ninst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ninst) {
+        insts[k].dtdx = dt[k] / insts[k].dx;
+        insts[k].dtdy = dt[k] / insts[k].dy;
+    }
+}
+
+// ------------------------------------------------------------------ //
+// layout conversion between the reference's padded AoS U(8,mx,my) /
+// V(10,mx,my) and the device SoA interior arrays
+
+__global__ void k_unpack(const Inst* __restrict__ insts, int inst, const double* __restrict__ src, int nv,
+                         int to_v0, int buf) {
+    const Inst& I = insts[inst];
+    const size_t plane = (size_t)I.nx * I.ny;
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane; o += (size_t)gridDim.x * blockDim.x) {
+        const int kj = (int)(o / I.nx), ki = (int)(o - (size_t)kj * I.nx);
+        const double* s = src + ((size_t)(kj + I.ngc) * I.mx + (ki + I.ngc)) * nv;
+        if (to_v0) {
+            // d, vx, vy, vz, p, bx, by, bz from V(1:8)
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) I.V0[v * plane + o] = s[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) I.U[buf][v * plane + o] = s[v];
+        }
+    }
+}
+
+// padded AoS U, V = cons2prim(U) and scheme, guards by the index map
+__global__ void k_pack(const Inst* __restrict__ insts, int inst, int buf, double* __restrict__ Uo,
+                       double* __restrict__ Vo, int32_t* __restrict__ So) {
+    const Inst& I = insts[inst];
+    const size_t n = (size_t)I.mx * I.my, plane = (size_t)I.nx * I.ny;
+    const Phys ph = phys_of(I);
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const int pj = (int)(p / I.mx), pi = (int)(p - (size_t)pj * I.mx);
+        const size_t o = (size_t)map_axis(pj, I.ngc, I.ny, I.bc) * I.nx + map_axis(pi, I.ngc, I.nx, I.bc);
+        double u[NCONS];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) u[v] = I.U[buf][v * plane + o];
+        if (Uo) {
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) Uo[p * NCONS + v] = u[v];
+        }
+        if (Vo) {
+            // src/convert.f90:198-234 including eint and gamma
+            double Bp = mag_pressure(ph, u);
+            double d = u[DENS], vx = u[MOMX] / d, vy = u[MOMY] / d, vz = u[MOMZ] / d;
+            double eint = (u[ENER] - Bp) / d - 0.5 * (vx * vx + vy * vy + vz * vz);
+            double pr = eos_pres(ph, d, eint);
+            double* V = Vo + p * 10;
+            V[0] = d;
+            V[1] = vx;
+            V[2] = vy;
+            V[3] = vz;
+            V[4] = pr;
+            V[5] = ph.hydro ? 0.0 : u[MAGX];
+            V[6] = ph.hydro ? 0.0 : u[MAGY];
+            V[7] = ph.hydro ? 0.0 : u[MAGZ];
+            V[8] = eint;
+            V[9] = ph.gamma;
+        }
+        if (So) So[p] = I.scheme[p];
+    }
+}
+
+// ------------------------------------------------------------------ //
+// host-side launchers
+
+static inline dim3 g1(int n) { return dim3((unsigned)n); }
+
+void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st) {
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_pv), pv, sizeof(double) * kPvSet, sizeof(double) * kPvSet * set,
+                                 hipMemcpyHostToDevice, st);
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_qw), qw, sizeof(double) * (kMaxR + 1) * 4,
+                                 sizeof(double) * (kMaxR + 1) * 4 * set, hipMemcpyHostToDevice, st);
+}
+
+void launch_stage_init(const LaunchSet& L, hipStream_t st) {
+    hipLaunchKernelGGL(k_stage_init, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
+}
+
+void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, hipStream_t st) {
+    hipLaunchKernelGGL(k_ctl, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst, op, stage);
+}
+
+void launch_face(const LaunchSet& L, int R, bool full, int in_buf, hipStream_t st) {
+    const dim3 b(kFaceTX * kFaceTY), g = g1(L.nface);
+#define HMHD_FACE(RR)                                                                     \
+    if (full)                                                                             \
+        hipLaunchKernelGGL((k_face<RR, true>), g, b, 0, st, L.insts, L.face, in_buf);     \
+    else                                                                                  \
+        hipLaunchKernelGGL((k_face<RR, false>), g, b, 0, st, L.insts, L.face, in_buf);
+    if (R == 1) {
+        HMHD_FACE(1)
+    } else if (R == 2) {
+        HMHD_FACE(2)
+    } else {
+        HMHD_FACE(3)
+    }
+#undef HMHD_FACE
+}
+
+void launch_recon_check(const LaunchSet& L, int R, int in_buf, hipStream_t st) {
+    if (R == 1)
+        hipLaunchKernelGGL(k_recon_check<1>, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, in_buf);
+    else if (R == 2)
+        hipLaunchKernelGGL(k_recon_check<2>, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, in_buf);
+    else
+        hipLaunchKernelGGL(k_recon_check<3>, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, in_buf);
+}
+
+void launch_lower_inner(const LaunchSet& L, hipStream_t st) {
+    hipLaunchKernelGGL(k_lower_inner, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
+}
+
+void launch_update(const LaunchSet& L, int in_buf, int out_buf, int base_buf, int stage, hipStream_t st) {
+    hipLaunchKernelGGL(k_update, g1(L.nupd), dim3(256), 0, st, L.insts, L.upd, in_buf, out_buf, base_buf, stage);
+}
+
+void launch_lower_detect(const LaunchSet& L, bool periodic_any, hipStream_t st) {
+    hipLaunchKernelGGL(k_lower_detect, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
+    if (periodic_any) {
+        hipLaunchKernelGGL(k_sync_x, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
+        hipLaunchKernelGGL(k_sync_y, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
+    }
+}
+
+void launch_cfl(const LaunchSet& L, int ninst, int buf, bool use_v0, hipStream_t st) {
+    hipLaunchKernelGGL(k_cfl_reset, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst);
+    hipLaunchKernelGGL(k_cfl, g1(L.ncell), dim3(256), 0, st, L.insts, L.cell, buf, use_v0 ? 1 : 0);
+}
+
+void launch_set_dt(Inst* insts, const double* dt, int ninst, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_dt, dim3((ninst + 63) / 64), dim3(64), 0, st, insts, dt, ninst);
+}
+
+void launch_unpack(const Inst* insts, int inst, const double* src, int nv, bool to_v0, int buf, size_t ncell,
+                   hipStream_t st) {
+    const int blocks = (int)((ncell + 255) / 256 < 4096 ? (ncell + 255) / 256 : 4096);
+    hipLaunchKernelGGL(k_unpack, dim3(blocks), dim3(256), 0, st, insts, inst, src, nv, to_v0 ? 1 : 0, buf);
+}
+
+void launch_pack(const Inst* insts, int inst, int buf, double* Uo, double* Vo, int32_t* So, size_t npad,
+                 hipStream_t st) {
+    const int blocks = (int)((npad + 255) / 256 < 4096 ? (npad + 255) / 256 : 4096);
+    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(256), 0, st, insts, inst, buf, Uo, Vo, So);
+}
+
+}  // namespace hmhd

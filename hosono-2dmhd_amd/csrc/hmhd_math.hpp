@@ -1,0 +1,227 @@
+// hmhd_math.hpp -- per-cell and per-face FP64 arithmetic of the step, on the
+// device.  Every expression keeps the reference's operand order so that the
+// library, built without multiply-add contraction, reproduces the reference
+// bit for bit:
+//   pressure floor          src/eos.f90:9-26
+//   cons -> prim            src/convert.f90:198-234
+//   unfloored pressure      src/convert.f90:236-254
+//   physical flux           src/convert.f90:28-92
+//   fast magnetosonic speed src/eigen.f90:257-321 (eigenvalues 1 and 7 only:
+//                           HLLC reads no other)
+//   HLLC-MHD (Li 2005)      src/riemannSolver.f90:68-221
+//   CAD / PAD               src/detect.f90:238-265
+// MAX(a,b) is a>b?a:b and MIN(a,b) is a<b?a:b, the rule the reference's
+// compiler applies.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hmhd_internal.hpp"
+
+namespace hmhd {
+
+enum : int { DENS = 0, MOMX = 1, MOMY = 2, MOMZ = 3, ENER = 4, MAGX = 5, MAGY = 6, MAGZ = 7 };
+
+
+struct Phys {
+    double gamma, small_p, small_rho;
+    int hydro;
+};
+
+__device__ __forceinline__ double fmaxF(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double fminF(double a, double b) { return a < b ? a : b; }
+
+// primitive state without the two constants eint/gamma the step never reads
+// from a Riemann state except through gamma itself
+struct Prim {
+    double d, vx, vy, vz, p, bx, by, bz;
+};
+
+__device__ __forceinline__ double eos_pres(const Phys& ph, double dens, double eint) {
+    double pres = (ph.gamma - 1.0) * dens * eint;
+    if (pres < ph.small_p) pres = ph.small_p;
+    return pres;
+}
+
+__device__ __forceinline__ double mag_pressure(const Phys& ph, const double* U) {
+    return ph.hydro ? 0.0 : 0.5 * (U[MAGX] * U[MAGX] + U[MAGY] * U[MAGY] + U[MAGZ] * U[MAGZ]);
+}
+
+__device__ __forceinline__ Prim cons2prim(const Phys& ph, const double* U) {
+    double Bp = mag_pressure(ph, U);
+    Prim V;
+    V.d = U[DENS];
+    V.vx = U[MOMX] / V.d;
+    V.vy = U[MOMY] / V.d;
+    V.vz = U[MOMZ] / V.d;
+    double eint = (U[ENER] - Bp) / V.d - 0.5 * (V.vx * V.vx + V.vy * V.vy + V.vz * V.vz);
+    V.p = eos_pres(ph, V.d, eint);
+    if (ph.hydro) {
+        V.bx = V.by = V.bz = 0.0;
+    } else {
+        V.bx = U[MAGX];
+        V.by = U[MAGY];
+        V.bz = U[MAGZ];
+    }
+    return V;
+}
+
+__device__ __forceinline__ double cons2pres(const Phys& ph, const double* U) {
+    double Bp = mag_pressure(ph, U);
+    return (ph.gamma - 1.0) *
+           (U[ENER] - Bp - 0.5 * (U[MOMX] * U[MOMX] + U[MOMY] * U[MOMY] + U[MOMZ] * U[MOMZ]) / U[DENS]);
+}
+
+__device__ __forceinline__ bool admissible(double dens, double pres) {
+    // CAD: no NaN/Inf; PAD: dens > 0 and pres > 0
+    bool cad = !(__builtin_isnan(dens) || __builtin_isnan(pres) || !__builtin_isfinite(dens) ||
+                 !__builtin_isfinite(pres));
+    return cad && (dens > 0.0) && (pres > 0.0);
+}
+
+__device__ __forceinline__ double eigen_cfs(double gam, double rho, double p, double BB, double BN) {
+    double cfs = (gam * p + BB) / rho;
+    cfs = 0.5 * (cfs + 1.0 * __builtin_sqrt(cfs * cfs - 4.0 * gam * p * (BN * BN) / (rho * rho)));
+    return __builtin_sqrt(fmaxF(1.0e-14, cfs));
+}
+
+// fast speed of the split system in direction DIR (0 = x, 1 = y)
+template <int DIR>
+__device__ __forceinline__ double fast_speed(const Phys& ph, const Prim& V) {
+    double rho = fmaxF(V.d, ph.small_rho);
+    double BN = DIR == 0 ? V.bx : V.by;
+    double p = fmaxF(V.p, ph.small_p);
+    double BB = V.bx * V.bx + V.by * V.by + V.bz * V.bz;
+    return eigen_cfs(ph.gamma, rho, p, BB, BN);
+}
+
+// indices of normal / transverse components for direction DIR
+template <int DIR>
+struct Dir {
+    static constexpr int mN = DIR == 0 ? MOMX : MOMY;
+    static constexpr int mT1 = DIR == 0 ? MOMY : MOMX;
+    static constexpr int mT2 = MOMZ;
+    static constexpr int bN = DIR == 0 ? MAGX : MAGY;
+    static constexpr int bT1 = DIR == 0 ? MAGY : MAGX;
+    static constexpr int bT2 = MAGZ;
+};
+
+__device__ __forceinline__ double pcomp(const Prim& V, int k) {
+    // velocity / field component by conservative index
+    switch (k) {
+        case MOMX: return V.vx;
+        case MOMY: return V.vy;
+        case MOMZ: return V.vz;
+        case MAGX: return V.bx;
+        case MAGY: return V.by;
+        default: return V.bz;
+    }
+}
+
+template <int DIR>
+__device__ __forceinline__ void cons2flux(const Phys& ph, const double* U, const Prim& V, double* F) {
+    using D = Dir<DIR>;
+    double Bp = mag_pressure(ph, U);
+    double E = U[ENER];
+    double Ptot = V.p + Bp;
+    const double vN = pcomp(V, D::mN), vT1 = pcomp(V, D::mT1), vT2 = pcomp(V, D::mT2);
+    const double bN = pcomp(V, D::bN), bT1 = pcomp(V, D::bT1), bT2 = pcomp(V, D::bT2);
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) F[v] = 0.0;
+    F[DENS] = U[D::mN];
+    F[D::mN] = U[D::mN] * U[D::mN] / U[DENS] + Ptot;
+    F[D::mT1] = U[D::mN] * vT1;
+    F[D::mT2] = U[D::mN] * vT2;
+    F[ENER] = (E + Ptot) * vN;
+    if (!ph.hydro) {
+        F[D::mN] = F[D::mN] - bN * bN;
+        F[D::mT1] = F[D::mT1] - bT1 * bN;
+        F[D::mT2] = F[D::mT2] - bT2 * bN;
+        F[D::bT1] = vN * bT1 - vT1 * bN;
+        F[D::bT2] = vN * bT2 - vT2 * bN;
+        F[ENER] = F[ENER] - bN * (V.vx * V.bx + V.vy * V.by + V.vz * V.bz);
+    }
+}
+
+template <int DIR>
+__device__ __forceinline__ void hllc(const Phys& ph, const double* uL, const double* uR, double* flux) {
+    using D = Dir<DIR>;
+    const Prim vL = cons2prim(ph, uL);
+    const Prim vR = cons2prim(ph, uR);
+    Prim vA;
+    vA.d = (vL.d + vR.d) / 2.0;
+    vA.vx = (vL.vx + vR.vx) / 2.0;
+    vA.vy = (vL.vy + vR.vy) / 2.0;
+    vA.vz = (vL.vz + vR.vz) / 2.0;
+    vA.p = (vL.p + vR.p) / 2.0;
+    vA.bx = (vL.bx + vR.bx) / 2.0;
+    vA.by = (vL.by + vR.by) / 2.0;
+    vA.bz = (vL.bz + vR.bz) / 2.0;
+    const double cL = fast_speed<DIR>(ph, vL);
+    const double cR = fast_speed<DIR>(ph, vR);
+    const double cA = fast_speed<DIR>(ph, vA);
+    const double vNL = pcomp(vL, D::mN), vNR = pcomp(vR, D::mN), vNA = pcomp(vA, D::mN);
+    const double sL = fminF(vNL - cL, vNA - cA);
+    const double sR = fmaxF(vNA + cA, vNR + cR);
+
+    double fL[NCONS], fR[NCONS], uHLL[NCONS];
+    cons2flux<DIR>(ph, uL, vL, fL);
+    cons2flux<DIR>(ph, uR, vR, fR);
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) uHLL[v] = (sR * uR[v] - sL * uL[v] + fL[v] - fR[v]) / (sR - sL);
+    const double velx = uHLL[MOMX] / uHLL[DENS], vely = uHLL[MOMY] / uHLL[DENS], velz = uHLL[MOMZ] / uHLL[DENS];
+    const double mdvHLL = uHLL[MAGX] * velx + uHLL[MAGY] * vely + uHLL[MAGZ] * velz;
+    const double mdvL = vL.bx * vL.vx + vL.by * vL.vy + vL.bz * vL.vz;
+    const double mdvR = vR.bx * vR.vx + vR.by * vR.vy + vR.bz * vR.vz;
+
+    const double velT1L = pcomp(vL, D::mT1), velT1R = pcomp(vR, D::mT1);
+    const double velT2L = pcomp(vL, D::mT2), velT2R = pcomp(vR, D::mT2);
+    const double momNL = uL[D::mN], momNR = uR[D::mN];
+    const double magNL = pcomp(vL, D::bN), magNR = pcomp(vR, D::bN), magNHLL = uHLL[D::bN];
+    const double magT1L = pcomp(vL, D::bT1), magT1R = pcomp(vR, D::bT1), magT1HLL = uHLL[D::bT1];
+    const double magT2L = pcomp(vL, D::bT2), magT2R = pcomp(vR, D::bT2), magT2HLL = uHLL[D::bT2];
+    const double rhoL = vL.d, rhoR = vR.d;
+    const double pTotL = vL.p + 0.5 * (vL.bx * vL.bx + vL.by * vL.by + vL.bz * vL.bz);
+    const double pTotR = vR.p + 0.5 * (vR.bx * vR.bx + vR.by * vR.by + vR.bz * vR.bz);
+    const double dL = sL - vNL, dR = sR - vNR;
+    const double sStr =
+        (momNR * dR - momNL * dL + pTotL - pTotR - magNL * magNL + magNR * magNR) / (rhoR * dR - rhoL * dL);
+    const double pStr = rhoL * dL * (sStr - vNL) + pTotL - magNL * magNL + magNHLL * magNHLL;
+
+    if (0.0 < sL) {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) flux[v] = fL[v];
+    } else if ((sL <= 0.0) && (0.0 < sStr)) {
+        double uS[NCONS];
+        uS[DENS] = rhoL * dL / (sL - sStr);
+        uS[D::mN] = uS[DENS] * sStr;
+        uS[D::mT1] = uS[DENS] * velT1L - (magNHLL * magT1HLL - magNL * magT1L) / (sL - sStr);
+        uS[D::mT2] = uS[DENS] * velT2L - (magNHLL * magT2HLL - magNL * magT2L) / (sL - sStr);
+        uS[D::bN] = magNHLL;
+        uS[D::bT1] = magT1HLL;
+        uS[D::bT2] = magT2HLL;
+        uS[ENER] = uL[ENER] * dL + pStr * sStr - pTotL * vNL - magNHLL * mdvHLL + magNL * mdvL;
+        uS[ENER] = uS[ENER] / (sL - sStr);
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) flux[v] = fL[v] + sL * (uS[v] - uL[v]);
+    } else if ((sStr <= 0.0) && (0.0 <= sR)) {
+        double uS[NCONS];
+        uS[DENS] = rhoR * dR / (sR - sStr);
+        uS[D::mN] = uS[DENS] * sStr;
+        uS[D::mT1] = uS[DENS] * velT1R - (magNHLL * magT1HLL - magNR * magT1R) / (sR - sStr);
+        uS[D::mT2] = uS[DENS] * velT2R - (magNHLL * magT2HLL - magNR * magT2R) / (sR - sStr);
+        uS[D::bN] = magNHLL;
+        uS[D::bT1] = magT1HLL;
+        uS[D::bT2] = magT2HLL;
+        uS[ENER] = uR[ENER] * dR + pStr * sStr - pTotR * vNR - magNHLL * mdvHLL + magNR * mdvR;
+        uS[ENER] = uS[ENER] / (sR - sStr);
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) flux[v] = fR[v] + sR * (uS[v] - uR[v]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) flux[v] = fR[v];
+    }
+}
+
+}  // namespace hmhd

@@ -1,0 +1,125 @@
+/* hmhd.h -- C ABI of the MI355X per-step update of hosono-2DMHD.
+ *
+ * The reference has no FFI: its hot path is two Fortran module procedures
+ * called from the driver's main loop,
+ *     dt = cfl_computedt(grid_block, grid_state)      src/mhd_driver.f90:78
+ *     call timeStep_advanceSolution(dt, grid_block, grid_state)
+ *                                                     src/mhd_driver.f90:81
+ * plus the set-up calls simulation_init / GP_init / grid_init
+ * (src/mhd_driver.f90:40-52).  The entry points below replace exactly those
+ * calls; a Fortran driver binds them with iso_c_binding.
+ *
+ * Array layouts are the reference's Fortran layouts on the padded grid
+ * (NGC = R + 2 guard cells per side, src/grid.f90:71-81):
+ *     U(8, mx, my), V(10, mx, my)   FP64, column-major, mx = nx + 2 NGC
+ *     scheme(mx, my)                int32
+ * Every function returns HMHD_OK (0) or an error code; hmhd_last_error()
+ * gives the message.  A handle is used by one host thread at a time; any
+ * number of handles may coexist.
+ */
+#ifndef HMHD_H
+#define HMHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HMHD_OK 0
+#define HMHD_ERR_ARG 1     /* invalid argument or configuration (the reference stops) */
+#define HMHD_ERR_DEVICE 2  /* HIP runtime error */
+#define HMHD_ERR_GP 3      /* GP covariance not positive definite (src/linAlgQuadPrecision.f90:136) */
+
+#define HMHD_BC_PERIODIC 0 /* sim_BC = 'periodic' */
+#define HMHD_BC_OUTFLOW 1  /* sim_BC = 'outflow'  */
+
+#define HMHD_RIEMANN_HLLC 1 /* sim_riemannSolver = 'hllc' | 'hllcmhd' (src/simulation.f90:62-72) */
+
+/* The par-file quantities the step reads (src/simulation.f90:43-58,
+ * src/grid.f90:58-63, src/GP.f90:58-69). */
+typedef struct hmhd_config {
+    int32_t nx, ny;             /* grid_Nx, grid_Ny */
+    double xbeg, xend;          /* grid_xBeg, grid_xEnd */
+    double ybeg, yend;          /* grid_yBeg, grid_yEnd */
+    int32_t R;                  /* GP_maxRadius, 1..3 */
+    double ell_over_delta;      /* GP_ellOverDelta */
+    double gamma;               /* sim_gamma */
+    double small_p, small_rho;  /* sim_smallPressure, sim_smallDensity */
+    double small_B;             /* sim_smallB (read by the reference, unused on the path) */
+    int32_t force_hydro;        /* sim_forceHydro */
+    int32_t bc;                 /* HMHD_BC_* */
+    double cfl;                 /* sim_cfl */
+    int32_t riemann;            /* HMHD_RIEMANN_HLLC */
+} hmhd_config;
+
+typedef struct hmhd_handle hmhd_t;
+
+/* Replaces simulation_init + GP_init + grid_init (src/mhd_driver.f90:40-52):
+ * validates cfg, builds the GP prediction vectors in binary128
+ * (src/GP.f90:37-200) and allocates the device state.  Requires
+ * nx, ny >= 2 (R + 2). */
+int hmhd_create(const hmhd_config* cfg, hmhd_t** out);
+void hmhd_destroy(hmhd_t* h);
+/* Message of the last failed call on h; with h == NULL, of the last failed
+ * hmhd_create on this thread. */
+const char* hmhd_last_error(const hmhd_t* h);
+
+/* NGC and the padded extents mx = nx + 2 NGC, my = ny + 2 NGC. */
+int hmhd_dims(const hmhd_t* h, int32_t* ngc, int32_t* mx, int32_t* my);
+
+/* GP tables in the reference's module layouts (src/GP.f90:12-21):
+ * predvec(nsMax, nqMax, 4, R), quadw(nqMax, R), stenc(2, nsMax, R), with
+ * nsMax = 1 + 2R(R+1), nqMax = R + 1.  Any pointer may be NULL. */
+int hmhd_get_gp(const hmhd_t* h, double* predvec, double* quadw, int32_t* stenc);
+/* The same tables straight from a configuration, without a device or a
+ * handle: the binary128 construction of GP_init (src/GP.f90:37-200). */
+int hmhd_gp_compute(const hmhd_config* cfg, double* predvec, double* quadw, int32_t* stenc);
+/* Replace the GP tables with ones the caller computed (same layouts). */
+int hmhd_set_gp(hmhd_t* h, const double* predvec, const double* quadw);
+
+/* Host -> device: the state initialCondition_set leaves in grid_state
+ * (src/initialCondition.f90:23-84).  U is required; V may be NULL.  When V
+ * is given, the next hmhd_cfl_dt reads it, as cfl_computedt reads the IC's
+ * primitives on the first step; otherwise it derives V from U. */
+int hmhd_upload_state(hmhd_t* h, const double* U, const double* V);
+
+/* Replaces cfl_computedt (src/cfl.f90:14-52). */
+int hmhd_cfl_dt(hmhd_t* h, double* dt);
+
+/* Replaces timeStep_advanceSolution (src/timeStep.f90:22-43): one SSP-RK3
+ * step of size dt with the full MOOD loop in every stage.  Enqueues and
+ * returns after the last MOOD pass has been decided; the state stays on
+ * the device. */
+int hmhd_advance(hmhd_t* h, double dt);
+
+/* Device -> host for output steps: U and V with guard cells as the
+ * reference holds them after a step (src/timeStep.f90:32-41), and the
+ * final scheme of the last stage (read by src/output.f90:251-253).  Any
+ * pointer may be NULL. */
+int hmhd_download(hmhd_t* h, double* U, double* V, int32_t* scheme);
+
+/* MOOD statistics of the last hmhd_advance: passes and inner
+ * admissibility iterations per RK stage (3 each). */
+int hmhd_stats(const hmhd_t* h, int32_t* passes, int32_t* inner);
+
+/* When on != 0, every later hmhd_advance records the final scheme of each
+ * of its three stages; hmhd_stage_schemes copies them out as
+ * scheme(mx, my, 3). */
+int hmhd_record_stage_schemes(hmhd_t* h, int32_t on);
+int hmhd_stage_schemes(const hmhd_t* h, int32_t* out);
+
+/* Kernel timing with HIP events on the handle's stream.  When on != 0,
+ * every launch is bracketed by events; hmhd_profile_read returns, per
+ * kernel family, the total milliseconds and launch counts since the last
+ * reset (order: face_full, face_flux, update, recon, flags, cfl). */
+int hmhd_profile(hmhd_t* h, int32_t on);
+int hmhd_profile_read(hmhd_t* h, double* ms, int64_t* launches, int32_t reset);
+
+/* Wait for all work enqueued on the handle. */
+int hmhd_synchronize(hmhd_t* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
