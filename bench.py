@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""FP64 zone-updates/s of the hosono-2DMHD step on one MI355X.
+
+Workload: the Orszag-Tang vortex on a 1024 x 1024 periodic grid (rho =
+gamma^2, p = gamma, v = (-sin 2 pi y, sin 2 pi x), B = (-sin 2 pi y,
+sin 4 pi x)/sqrt(4 pi), gamma = 5/3), GP-MOOD radius 2 with l = 12 dx, HLLC,
+SSP-RK3, CFL 0.8, one instance, state resident in HBM.  A step is one
+driver iteration: the CFL reduction (cfl_computedt) and one full SSP-RK3
+step with every MOOD pass (timeStep_advanceSolution).  A zone-update is one
+interior cell advanced one step.
+
+With no --steps the timed region is the whole solve from t = 0 to t = 0.5;
+with --steps K it is exactly K steps from t = 0.  Warm-up steps run on a
+separate copy before the timed region.
+
+One JSON line on stdout.  `roofline` covers the dominant kernel, the face
+kernel in its first-pass form (GP reconstruction + admissibility + HLLC),
+timed with HIP events on the solver's stream inside the timed region.
+`cpu_baseline` times the reference's own Fortran step (compiled from its
+sources by oracle/Makefile, OpenMP on the host cores) on a bounded sample of
+the same workload, in a child process started before the GPU is touched.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import pathlib
+import subprocess
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+PKG_DIR = ROOT / "hosono-2dmhd_amd"
+
+METRIC = "FP64 zone-updates/s at 1024^2 (plain vs batched); served p50/p99 time-to-solution"
+UNIT = "zone-updates/s"
+
+# MI355X vendor peaks
+PEAK_FP64_TFLOPS = 78.6  # FP64 vector == FP64 matrix on CDNA4
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic FP64 flops (+ - * / sqrt = 1; min/max/abs/compare = 0) of the
+# first MOOD pass at GP radius 2, counted from the arithmetic each kernel
+# performs; see the breakdown in DESIGN.md.
+FLOP_CONS2PRIM = 21
+FLOP_CONS2PRES = 17
+FLOP_FAST_SPEED = 21
+FLOP_CONS2FLUX = 33      # given the primitives
+FLOP_HLLC_STAR = 2 * FLOP_CONS2PRIM + 20 + 3 * FLOP_FAST_SPEED + 4 + 2 * FLOP_CONS2FLUX + 49 + 18 + 14 + 2 + 13 + 8 + 27 + 24
+FLOP_RECON_R2 = 4 * 3 * 13 * 8 * 2      # 4 face sides x 3 points x 13 cells x 8 vars, mul + add
+FLOP_ADMISSIBILITY_R2 = 4 * 3 * FLOP_CONS2PRES
+FLOP_FLUX_SUM_R2 = 2 * 3 * 16           # 2 owned faces x 3 points x (8 mul + 8 add)
+FLOP_FACE_KERNEL_R2 = FLOP_RECON_R2 + FLOP_ADMISSIBILITY_R2 + 2 * 3 * FLOP_HLLC_STAR + FLOP_FLUX_SUM_R2
+FLOP_UPDATE_KERNEL = 2 * 8 + 8 * 6 + (FLOP_CONS2PRES + 5 * FLOP_CONS2PRIM + 7 + 7 + 5)
+FLOP_RK_COMBINE = 2 * 8 * 3
+FLOP_CFL = FLOP_CONS2PRIM + 5 + 16 + 2
+FLOP_ZONE_UPDATE_R2 = 3 * (FLOP_FACE_KERNEL_R2 + FLOP_UPDATE_KERNEL) + FLOP_RK_COMBINE + FLOP_CFL
+# compulsory HBM bytes of the face kernel per interior cell: read U (8 FP64),
+# write the raw x and y fluxes (16 FP64) and the two FLICD components (2 FP64)
+BYTES_FACE_KERNEL = 8 * 8 + 16 * 8 + 2 * 8
+
+
+def load_package():
+    if "hosono_2dmhd_amd" in sys.modules:
+        return sys.modules["hosono_2dmhd_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "hosono_2dmhd_amd", PKG_DIR / "__init__.py", submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["hosono_2dmhd_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def workload(n: int):
+    load_package()
+    import hosono_2dmhd_amd.ics as ics
+    g = ics.Grid(n, n, 0.0, 1.0, 0.0, 1.0, R=2)
+    U, V = ics.orszag_tang(g)
+    return g, U, V
+
+
+def cpu_baseline_child(n: int, steps: int) -> None:
+    """Runs in a child process: the reference's Fortran step on the host."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    g, U, V = workload(n)
+    cfg = O.Config(n, n, R=2, ell_over_delta=12.0)
+    if O.ref_available("fast"):
+        kind = "reference"
+        # the Fortran runtime prints banners on fd 1: keep stdout for the result
+        saved = os.dup(1)
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(devnull, 1)
+        try:
+            ref = O.RefHarness(cfg, "fast")
+            ref.set_state(U, V)
+            ref.cfl_dt()
+            t0 = time.perf_counter()
+            ref.run(steps, 1.0e30)
+            el = time.perf_counter() - t0
+            ref.flush()
+        finally:
+            os.dup2(saved, 1)
+            os.close(devnull)
+    else:
+        kind = "port"
+        orc = O.Oracle(cfg)
+        orc.set_state(U, V)
+        t0 = time.perf_counter()
+        orc.run(steps, 1.0e30)
+        el = time.perf_counter() - t0
+    print(json.dumps({"value": steps * n * n / el, "unit": UNIT, "cores": threads, "kind": kind,
+                      "sample": f"{steps} steps of the {n}x{n} Orszag-Tang solve from t=0 "
+                                f"({'reference Fortran, amdflang -O3 -fopenmp' if kind == 'reference' else 'C restatement, OpenMP'}), "
+                                f"{el:.2f} s"}))
+
+
+def _big_stack():
+    # the reference keeps its three RK stage arrays as automatic arrays
+    # (src/timeStep.f90:154-157): 3 x 68 MB on the stack at 1024^2
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
+def run_cpu_baseline(n: int, steps: int):
+    try:
+        p = subprocess.run([sys.executable, str(pathlib.Path(__file__).resolve()), "--cpu-child",
+                            "--n", str(n), "--cpu-steps", str(steps)],
+                           capture_output=True, text=True, timeout=600, preexec_fn=_big_stack)
+        line = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+        if p.returncode != 0 or not line:
+            return {"value": None, "unit": UNIT, "cores": None, "kind": None,
+                    "sample": f"failed: {p.stderr.strip()[-300:]}"}
+        return json.loads(line[-1])
+    except subprocess.TimeoutExpired:
+        return {"value": None, "unit": UNIT, "cores": None, "kind": None, "sample": "timed out"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the whole solve to t=0.5)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--tmax", type=float, default=0.5)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
+    args = ap.parse_args()
+
+    if args.cpu_child:
+        cpu_baseline_child(args.n, args.cpu_steps)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this one-GPU build "
+              f"(independent solves shard by instance; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    cpu = None if args.no_cpu else run_cpu_baseline(args.n, args.cpu_steps)
+
+    load_package()
+    import hosono_2dmhd_amd.runtime as rt
+    g, U, V = workload(args.n)
+    cfg = rt.Config(args.n, args.n, R=2, ell_over_delta=12.0)
+
+    # warm-up on a throwaway copy (module load, clocks, first launches)
+    with rt.Solver(cfg) as w:
+        w.upload(U, V)
+        for _ in range(args.warmup):
+            w.advance_solution(w.cfl_computedt())
+        w.synchronize()
+
+    s = rt.Solver(cfg)
+    s.upload(U, V)
+    s.profile(True)
+    s.synchronize()
+    s.profile_read(reset=True)
+    tmax = args.tmax if args.steps == 0 else float("inf")
+    limit = args.steps if args.steps > 0 else 10_000_000
+    t = 0.0
+    steps = 0
+    passes = 0
+    t0 = time.perf_counter()
+    while t < tmax and steps < limit:
+        dt = rt.valid_time_step(s.cfl_computedt(), t, tmax)
+        s.advance_solution(dt)
+        t += dt
+        steps += 1
+        passes += int(s.stats()[0].sum())
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = s.profile_read()
+    s.close()
+
+    zones = args.n * args.n
+    value = steps * zones / elapsed
+    ms_full, n_full = prof["face_full"]
+    avg_s = ms_full / n_full / 1e3 if n_full else float("nan")
+    flops_launch = zones * FLOP_FACE_KERNEL_R2
+    achieved = flops_launch / avg_s / 1e12
+    traffic = None
+    tf = ROOT / "profiles" / "face_kernel_traffic_ot1024.json"
+    if tf.exists() and args.n == 1024:
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    roofline = {
+        "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+        "kernel": "k_face<2,FULL>", "flops_per_launch": flops_launch, "avg_launch_ms": avg_s * 1e3,
+        "launches": n_full, "peak_note": "FP64 vector peak (equal to the FP64 MFMA peak on MI355X)",
+        "hbm_achieved_gbs": zones * BYTES_FACE_KERNEL / avg_s / 1e9, "hbm_peak_gbs": PEAK_HBM_GBS,
+    }
+    line = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"orszag_tang_{args.n}", "nx": args.n, "ny": args.n, "gp_radius": 2,
+                   "ell_over_delta": 12.0, "riemann": "hllc", "cfl": 0.8, "t_end": t,
+                   "form": "plain", "mood_passes_per_stage": passes / (3 * steps)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernel_ms": {k: v[0] for k, v in prof.items()},
+        "kernel_launches": {k: v[1] for k, v in prof.items()},
+        "flops_per_zone_update": FLOP_ZONE_UPDATE_R2,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -278,6 +278,7 @@ class RefHarness:
             lib.ref_run.argtypes = [C.c_int, C.c_double]
             lib.ref_reconstruct_all.argtypes = [C.c_int, dp]
             lib.ref_step_masks.argtypes = [C.c_double, dp, ip, ip]
+            lib.ref_flush.argtypes = []
             RefHarness._libs[variant] = lib
         self.lib = RefHarness._libs[variant]
         self._tmp = tempfile.TemporaryDirectory()
@@ -288,6 +289,9 @@ class RefHarness:
         d = [C.c_int() for _ in range(6)]
         self.lib.ref_dims(*[C.byref(x) for x in d])
         self.ngc, self.mx, self.my, self.nsmax, self.nqmax, self.rmax = (x.value for x in d)
+
+    def flush(self) -> None:
+        self.lib.ref_flush()
 
     def gp_tables(self):
         pv = np.zeros((self.rmax, 4, self.nqmax, self.nsmax))

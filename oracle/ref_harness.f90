@@ -61,6 +61,11 @@ contains
         ref_ready = .false.
     end subroutine ref_finalize
 
+    ! flush the Fortran runtime's buffered stdout (the reference's banners)
+    subroutine ref_flush() bind(C, name='ref_flush')
+        flush(6)
+    end subroutine ref_flush
+
     subroutine ref_dims(ngc, mx, my, nsmax, nqmax, rmax) bind(C, name='ref_dims')
         integer(c_int), intent(out) :: ngc, mx, my, nsmax, nqmax, rmax
         ngc = grid_block%NGC
