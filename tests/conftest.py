@@ -22,8 +22,9 @@ def load_package():
 load_package()
 if str(ROOT / "oracle") not in sys.path:
     sys.path.insert(0, str(ROOT / "oracle"))
-if str(ROOT / "tests") not in sys.path:
-    sys.path.insert(0, str(ROOT / "tests"))
+for extra in (ROOT / "tests", ROOT / "tests" / "golden"):
+    if str(extra) not in sys.path:
+        sys.path.insert(0, str(extra))
 
 
 def pytest_configure(config):

@@ -128,3 +128,28 @@ def test_errors_are_reported():
     s = rt.Solver(rt.Config(32, 32))
     with pytest.raises(ValueError):
         s.upload(np.zeros((10, 10, 8)))
+
+
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+@pytest.mark.parametrize("name", sorted(p.stem[4:] for p in GOLDEN.glob("ref_*.npz")))
+def test_gpu_reproduces_reference_fixture(name):
+    """The reference's own Fortran step (tests/golden/make_golden.py) on the
+    same inputs: every dt, the final U and V, the stage masks of the last
+    step, bit for bit."""
+    fx = np.load(GOLDEN / f"ref_{name}.npz")
+    n = int(fx["n"])
+    cfg, g, U, V = cases.make(name, n=n if n > 0 else None)
+    gpu = rt.Solver(cases.runtime_config(cfg))
+    gpu.upload(U, V)
+    dts = fx["dts"]
+    for k, dt_ref in enumerate(dts):
+        dt = gpu.cfl_computedt()
+        assert dt == dt_ref, f"dt of step {k + 1}"
+        gpu.record_stage_schemes(k == len(dts) - 1)
+        gpu.advance_solution(dt)
+    Ug, Vg, Sg = gpu.download()
+    assert np.array_equal(Ug, fx["U"]) and np.array_equal(Vg, fx["V"])
+    assert np.array_equal(gpu.stage_schemes().astype(np.int8), fx["stage_schemes"])
+    assert np.array_equal(gpu.stats()[0], fx["passes"])
