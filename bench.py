@@ -43,7 +43,7 @@ PEAK_HBM_GBS = 8000.0
 
 # Algorithmic FP64 flops (+ - * / sqrt = 1; min/max/abs/compare = 0) of the
 # first MOOD pass at GP radius 2, counted from the arithmetic each kernel
-# performs; see the breakdown in DESIGN.md.
+# performs, summed per zone below.
 FLOP_CONS2PRIM = 21
 FLOP_CONS2PRES = 17
 FLOP_FAST_SPEED = 21
@@ -157,7 +157,8 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this one-GPU build "
-              f"(independent solves shard by instance; see DESIGN.md)", file=sys.stderr)
+              f"(a multi-GPU form would shard a solve by row bands with guard-row exchange, "
+              f"or place independent solves on separate devices)", file=sys.stderr)
         return 2
 
     cpu = None if args.no_cpu else run_cpu_baseline(args.n, args.cpu_steps)

@@ -1,78 +1,52 @@
 // hmhd_api.cpp -- the extern "C" entry points declared in include/hmhd.h:
-// a single-run handle is an Engine holding one instance.
+// the single-run handle (one solve in an Engine), the batch (many solves
+// advanced together to their own end times) and the server (a request loop
+// that admits solves by HBM budget while others run).
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 
 #include "../../include/hmhd.h"
 #include "gp_tables.hpp"
+#include "hmhd_runner.hpp"
 #include "hmhd_solver.hpp"
 
 struct hmhd_handle {
     hmhd::Engine eng;
+    int64_t id = -1;
+    std::string err;
+};
+
+struct hmhd_batch_handle {
+    hmhd::Engine eng;
+    hmhd::Runner run{eng};
     std::string err;
 };
 
 namespace {
 thread_local std::string g_create_err;
 
-int fail(hmhd_t* h, int code, const std::string& msg) {
-    if (h) h->err = msg;
+int fail(std::string& where, int code, const std::string& msg) {
+    where = msg;
     return code;
 }
 
-int pass(hmhd_t* h, int code) {
-    if (code != HMHD_OK) h->err = h->eng.err;
-    return code;
-}
-}  // namespace
-
-extern "C" {
-
-int hmhd_create(const hmhd_config* cfg, hmhd_t** out) {
-    if (!cfg || !out) {
-        g_create_err = "hmhd_create: null argument";
-        return HMHD_ERR_ARG;
-    }
-    *out = nullptr;
-    hmhd_t* h = new hmhd_handle;
-    if (h->eng.add_instance(*cfg) < 0) {
-        g_create_err = h->eng.err;
-        const int code = g_create_err.find("positive definite") != std::string::npos ? HMHD_ERR_GP : HMHD_ERR_ARG;
-        delete h;
-        return code;
-    }
-    int rc = h->eng.finalize();
-    if (rc != HMHD_OK) {
-        g_create_err = h->eng.err;
-        delete h;
-        return rc;
-    }
-    *out = h;
-    return HMHD_OK;
+int create_code(const std::string& msg) {
+    if (msg.find("positive definite") != std::string::npos) return HMHD_ERR_GP;
+    if (msg.find("hip") != std::string::npos || msg.find("constant memory") != std::string::npos)
+        return HMHD_ERR_DEVICE;
+    return HMHD_ERR_ARG;
 }
 
-void hmhd_destroy(hmhd_t* h) { delete h; }
-
-const char* hmhd_last_error(const hmhd_t* h) { return h ? h->err.c_str() : g_create_err.c_str(); }
-
-int hmhd_dims(const hmhd_t* h, int32_t* ngc, int32_t* mx, int32_t* my) {
-    if (!h) return HMHD_ERR_ARG;
-    const auto& in = h->eng.inst[0];
-    if (ngc) *ngc = in.ngc;
-    if (mx) *mx = in.mx;
-    if (my) *my = in.my;
-    return HMHD_OK;
-}
-
-}  // extern "C"
-
-namespace {
 // device table layout -> the reference's GP_predictionVectors(nsMax, nqMax,
 // 4, R), GP_quadratureWeights(nqMax, R), GP_stencIdxs(2, nsMax, R)
-void export_gp(int R, const double* pv_dev, const double* qw_dev, double* predvec, double* quadw, int32_t* stenc) {
+void export_gp(int R, const double* pv, const double* qw, double* predvec, double* quadw, int32_t* stenc) {
     const int nsm = hmhd::gp_ns(R), nqm = hmhd::gp_nq(R);
-    const double* pv = pv_dev;
-    const double* qw = qw_dev;
     if (predvec) {
         std::memset(predvec, 0, sizeof(double) * nsm * nqm * 4 * R);
         for (int rr = 1; rr <= R; rr++)
@@ -101,41 +75,227 @@ void export_gp(int R, const double* pv_dev, const double* qw_dev, double* predve
         }
     }
 }
+
+int profile_read(hmhd::Engine& e, double* ms, int64_t* launches, int32_t reset) {
+    for (int f = 0; f < hmhd::PF_N; f++) {
+        if (ms) ms[f] = e.prof_ms[f];
+        if (launches) launches[f] = e.prof_n[f];
+        if (reset) {
+            e.prof_ms[f] = 0.0;
+            e.prof_n[f] = 0;
+        }
+    }
+    return HMHD_OK;
+}
 }  // namespace
+
+// ------------------------------------------------------------------ //
+// the request loop of the served form
+
+struct hmhd_server_handle {
+    struct Req {
+        int64_t id = 0;
+        hmhd_config cfg{};
+        std::vector<double> U, V;
+        double tmax = 0.0;
+        int32_t nstepmax = 0;
+        size_t bytes = 0;
+        int64_t solve = -1;
+        hmhd_result res{};
+        std::vector<double> Uout, Vout;
+        std::vector<int32_t> Sout;
+        bool done = false;
+    };
+
+    hmhd::Engine eng;  // touched by the worker thread only
+    hmhd::Runner run{eng};
+    uint64_t budget = 0;
+    int32_t max_active = 0;
+    std::mutex mu;  // guards everything below
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;
+    std::unordered_map<int64_t, Req> reqs;
+    std::unordered_map<int64_t, int64_t> by_solve;  // solve id -> request id
+    int64_t next_req = 1;
+    bool stop = false;
+    std::string err;
+    std::thread worker;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    hmhd_server_stats st{};
+    double busy_s = 0.0;
+
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+
+    void finish_locked(Req& r, int32_t state) {
+        r.res.state = state;
+        r.res.done_s = now();
+        r.done = true;
+    }
+
+    // Admit queued requests in arrival order while they fit the budget and
+    // the slot limit; a request that does not fit yet is passed over for
+    // later, smaller ones.  One larger than the whole budget fails at once.
+    void admit_locked() {
+        bool changed = false;
+        for (auto it = queue.begin(); it != queue.end();) {
+            Req& r = reqs[*it];
+            if (r.bytes > budget) {
+                err = "request footprint exceeds the HBM budget";
+                finish_locked(r, HMHD_REQ_FAILED);
+                it = queue.erase(it);
+                changed = true;
+                continue;
+            }
+            const bool slot = max_active <= 0 || (int32_t)by_solve.size() < max_active;
+            if (!slot) break;
+            if (eng.bytes_in_use() + r.bytes > budget) {
+                ++it;
+                continue;
+            }
+            const int64_t sid = eng.add(r.cfg);
+            if (sid < 0 || eng.upload(sid, r.U.data(), r.V.empty() ? nullptr : r.V.data()) != HMHD_OK) {
+                err = eng.err;
+                if (sid >= 0) (void)eng.remove(sid);
+                finish_locked(r, HMHD_REQ_FAILED);
+                it = queue.erase(it);
+                changed = true;
+                continue;
+            }
+            r.solve = sid;
+            r.res.state = HMHD_REQ_RUNNING;
+            r.res.admit_s = now();
+            std::vector<double>().swap(r.U);
+            std::vector<double>().swap(r.V);
+            run.track(sid, r.tmax, r.nstepmax);
+            by_solve[sid] = r.id;
+            it = queue.erase(it);
+        }
+        st.active = (int32_t)by_solve.size();
+        st.queued = (int32_t)queue.size();
+        if (st.active > st.peak_active) st.peak_active = st.active;
+        st.bytes_in_use = eng.bytes_in_use();
+        if (st.bytes_in_use > st.peak_bytes) st.peak_bytes = st.bytes_in_use;
+        if (changed) cv_done.notify_all();
+    }
+
+    void loop() {
+        std::vector<int64_t> fin;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                admit_locked();
+                // with nothing admitted every queued request fits an empty
+                // engine or has failed, so this wait cannot spin
+                cv_work.wait(lk, [&] { return stop || !by_solve.empty() || !queue.empty(); });
+                if (stop) return;
+                admit_locked();
+                if (by_solve.empty()) continue;
+            }
+            // one engine step of every admitted run, outside the lock so
+            // submissions and polls go on meanwhile
+            fin.clear();
+            const double a = now();
+            const int rc = run.step(fin);
+            const double b = now();
+            std::lock_guard<std::mutex> lk(mu);
+            busy_s += b - a;
+            st.engine_steps += 1;
+            if (rc != HMHD_OK) {
+                // a device error leaves every running request undefined
+                err = eng.err;
+                for (auto& kv : by_solve) {
+                    finish_locked(reqs[kv.second], HMHD_REQ_FAILED);
+                    (void)eng.remove(kv.first);
+                    run.forget(kv.first);
+                }
+                by_solve.clear();
+                cv_done.notify_all();
+                continue;
+            }
+            for (int64_t sid : fin) {
+                Req& r = reqs[by_solve[sid]];
+                const hmhd::RunInfo& ri = run.info[sid];
+                const hmhd::InstHost* h = eng.find(sid);
+                const size_t np = (size_t)h->mx * h->my;
+                r.Uout.resize(8 * np);
+                r.Vout.resize(10 * np);
+                r.Sout.resize(np);
+                int32_t state = ri.state == hmhd::RUN_DONE ? HMHD_REQ_DONE : HMHD_REQ_ABORTED;
+                if (eng.download(sid, r.Uout.data(), r.Vout.data(), r.Sout.data()) != HMHD_OK) {
+                    err = eng.err;
+                    state = HMHD_REQ_FAILED;
+                }
+                r.res.steps = ri.steps;
+                r.res.t = ri.t;
+                r.res.zone_updates = ri.zone_updates;
+                finish_locked(r, state);
+                st.completed += 1;
+                st.zone_updates += ri.zone_updates;
+                (void)eng.remove(sid);
+                run.forget(sid);
+                by_solve.erase(sid);
+            }
+            if (!fin.empty()) cv_done.notify_all();
+        }
+    }
+};
 
 extern "C" {
 
+// ------------------------------------------------------------------ //
+// single run
+
+int hmhd_create(const hmhd_config* cfg, hmhd_t** out) {
+    if (!cfg || !out) return fail(g_create_err, HMHD_ERR_ARG, "hmhd_create: null argument");
+    *out = nullptr;
+    hmhd_t* h = new hmhd_handle;
+    h->id = h->eng.add(*cfg);
+    if (h->id < 0) {
+        g_create_err = h->eng.err;
+        delete h;
+        return create_code(g_create_err);
+    }
+    *out = h;
+    return HMHD_OK;
+}
+
+void hmhd_destroy(hmhd_t* h) { delete h; }
+
+const char* hmhd_last_error(const hmhd_t* h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+int hmhd_dims(const hmhd_t* h, int32_t* ngc, int32_t* mx, int32_t* my) {
+    if (!h) return HMHD_ERR_ARG;
+    const hmhd::InstHost* in = h->eng.find(h->id);
+    if (ngc) *ngc = in->ngc;
+    if (mx) *mx = in->mx;
+    if (my) *my = in->my;
+    return HMHD_OK;
+}
+
 int hmhd_gp_compute(const hmhd_config* cfg, double* predvec, double* quadw, int32_t* stenc) {
-    if (!cfg) {
-        g_create_err = "hmhd_gp_compute: null config";
-        return HMHD_ERR_ARG;
-    }
-    std::string err;
-    if (hmhd::Engine::check_config(*cfg, err) != HMHD_OK) {
-        g_create_err = err;
-        return HMHD_ERR_ARG;
-    }
+    if (!cfg) return fail(g_create_err, HMHD_ERR_ARG, "hmhd_gp_compute: null config");
+    if (hmhd::Engine::check_config(*cfg, g_create_err) != HMHD_OK) return HMHD_ERR_ARG;
     double pv[hmhd::kPvSet], qw[(hmhd::kMaxR + 1) * 4];
     if (hmhd::gp_compute(cfg->R, cfg->ell_over_delta, cfg->xbeg, cfg->xend, cfg->ybeg, cfg->yend, cfg->nx, cfg->ny,
-                         pv, qw) != 0) {
-        g_create_err = "GP covariance matrix is not positive definite (src/linAlgQuadPrecision.f90:136)";
-        return HMHD_ERR_GP;
-    }
+                         pv, qw) != 0)
+        return fail(g_create_err, HMHD_ERR_GP,
+                    "GP covariance matrix is not positive definite (src/linAlgQuadPrecision.f90:136)");
     export_gp(cfg->R, pv, qw, predvec, quadw, stenc);
     return HMHD_OK;
 }
 
 int hmhd_get_gp(const hmhd_t* h, double* predvec, double* quadw, int32_t* stenc) {
     if (!h) return HMHD_ERR_ARG;
-    const auto& in = h->eng.inst[0];
-    export_gp(in.cfg.R, in.pv, in.qw, predvec, quadw, stenc);
+    const hmhd::InstHost* in = h->eng.find(h->id);
+    export_gp(in->cfg.R, in->pv, in->qw, predvec, quadw, stenc);
     return HMHD_OK;
 }
 
 int hmhd_set_gp(hmhd_t* h, const double* predvec, const double* quadw) {
-    if (!h || !predvec || !quadw) return fail(h, HMHD_ERR_ARG, "hmhd_set_gp: null argument");
-    const auto& in = h->eng.inst[0];
-    const int R = in.cfg.R, nsm = hmhd::gp_ns(R), nqm = hmhd::gp_nq(R);
+    if (!h) return HMHD_ERR_ARG;
+    if (!predvec || !quadw) return fail(h->err, HMHD_ERR_ARG, "hmhd_set_gp: null argument");
+    const hmhd::InstHost* in = h->eng.find(h->id);
+    const int R = in->cfg.R, nsm = hmhd::gp_ns(R), nqm = hmhd::gp_nq(R);
     double pv[hmhd::kPvSet] = {0};
     double qw[(hmhd::kMaxR + 1) * 4] = {0};
     for (int rr = 1; rr <= R; rr++) {
@@ -146,38 +306,49 @@ int hmhd_set_gp(hmhd_t* h, const double* predvec, const double* quadw) {
                         predvec[(((size_t)(rr - 1) * 4 + f) * nqm + q) * nsm + k];
         for (int q = 0; q < hmhd::gp_nq(rr); q++) qw[rr * 4 + q] = quadw[(rr - 1) * nqm + q];
     }
-    return pass(h, h->eng.set_gp(0, pv, qw));
+    const int rc = h->eng.set_gp(h->id, pv, qw);
+    if (rc) h->err = h->eng.err;
+    return rc;
 }
 
 int hmhd_upload_state(hmhd_t* h, const double* U, const double* V) {
-    if (!h || !U) return fail(h, HMHD_ERR_ARG, "hmhd_upload_state: U is required");
-    return pass(h, h->eng.upload(0, U, V));
+    if (!h) return HMHD_ERR_ARG;
+    if (!U) return fail(h->err, HMHD_ERR_ARG, "hmhd_upload_state: U is required");
+    const int rc = h->eng.upload(h->id, U, V);
+    if (rc) h->err = h->eng.err;
+    return rc;
 }
 
 int hmhd_cfl_dt(hmhd_t* h, double* dt) {
-    if (!h || !dt) return fail(h, HMHD_ERR_ARG, "hmhd_cfl_dt: null argument");
+    if (!h) return HMHD_ERR_ARG;
+    if (!dt) return fail(h->err, HMHD_ERR_ARG, "hmhd_cfl_dt: null argument");
     std::vector<double> d;
-    int rc = h->eng.cfl(d);
-    if (rc == HMHD_OK) *dt = d[0];
-    return pass(h, rc);
+    const int rc = h->eng.cfl(d);
+    if (rc) h->err = h->eng.err;
+    else *dt = d[0];
+    return rc;
 }
 
 int hmhd_advance(hmhd_t* h, double dt) {
     if (!h) return HMHD_ERR_ARG;
-    return pass(h, h->eng.advance(&dt));
+    const int rc = h->eng.advance(std::vector<double>{dt});
+    if (rc) h->err = h->eng.err;
+    return rc;
 }
 
 int hmhd_download(hmhd_t* h, double* U, double* V, int32_t* scheme) {
     if (!h) return HMHD_ERR_ARG;
-    return pass(h, h->eng.download(0, U, V, scheme));
+    const int rc = h->eng.download(h->id, U, V, scheme);
+    if (rc) h->err = h->eng.err;
+    return rc;
 }
 
 int hmhd_stats(const hmhd_t* h, int32_t* passes, int32_t* inner) {
     if (!h) return HMHD_ERR_ARG;
-    const int32_t* c = h->eng.ctl_host.data();
+    const hmhd::InstHost* in = h->eng.find(h->id);
     for (int s = 0; s < 3; s++) {
-        if (passes) passes[s] = c[hmhd::CTL_PASSES + s];
-        if (inner) inner[s] = c[hmhd::CTL_INNER + s];
+        if (passes) passes[s] = in->passes[s];
+        if (inner) inner[s] = in->inner[s];
     }
     return HMHD_OK;
 }
@@ -190,7 +361,7 @@ int hmhd_record_stage_schemes(hmhd_t* h, int32_t on) {
 
 int hmhd_stage_schemes(const hmhd_t* h, int32_t* out) {
     if (!h || !out) return HMHD_ERR_ARG;
-    const auto& v = h->eng.stage_schemes[0];
+    const auto& v = h->eng.find(h->id)->stage_schemes;
     if (v.empty()) return HMHD_ERR_ARG;
     std::memcpy(out, v.data(), sizeof(int32_t) * v.size());
     return HMHD_OK;
@@ -204,20 +375,202 @@ int hmhd_profile(hmhd_t* h, int32_t on) {
 
 int hmhd_profile_read(hmhd_t* h, double* ms, int64_t* launches, int32_t reset) {
     if (!h) return HMHD_ERR_ARG;
-    for (int f = 0; f < hmhd::PF_N; f++) {
-        if (ms) ms[f] = h->eng.prof_ms[f];
-        if (launches) launches[f] = h->eng.prof_n[f];
-        if (reset) {
-            h->eng.prof_ms[f] = 0.0;
-            h->eng.prof_n[f] = 0;
+    return profile_read(h->eng, ms, launches, reset);
+}
+
+int hmhd_synchronize(hmhd_t* h) {
+    if (!h) return HMHD_ERR_ARG;
+    const int rc = h->eng.sync();
+    if (rc) h->err = h->eng.err;
+    return rc;
+}
+
+// ------------------------------------------------------------------ //
+// batch
+
+int hmhd_batch_create(hmhd_batch_t** out) {
+    if (!out) return fail(g_create_err, HMHD_ERR_ARG, "hmhd_batch_create: null argument");
+    *out = new hmhd_batch_handle;
+    return HMHD_OK;
+}
+
+void hmhd_batch_destroy(hmhd_batch_t* b) { delete b; }
+
+const char* hmhd_batch_last_error(const hmhd_batch_t* b) { return b ? b->err.c_str() : g_create_err.c_str(); }
+
+int hmhd_batch_add(hmhd_batch_t* b, const hmhd_config* cfg, const double* U, const double* V, double tmax,
+                   int32_t nstepmax, int64_t* id) {
+    if (!b) return HMHD_ERR_ARG;
+    if (!cfg || !U || !id) return fail(b->err, HMHD_ERR_ARG, "hmhd_batch_add: null argument");
+    const int64_t sid = b->eng.add(*cfg);
+    if (sid < 0) return fail(b->err, create_code(b->eng.err), b->eng.err);
+    const int rc = b->eng.upload(sid, U, V);
+    if (rc) {
+        b->err = b->eng.err;
+        (void)b->eng.remove(sid);
+        return rc;
+    }
+    b->run.track(sid, tmax, nstepmax);
+    *id = sid;
+    return HMHD_OK;
+}
+
+int hmhd_batch_run(hmhd_batch_t* b, int32_t max_steps) {
+    if (!b) return HMHD_ERR_ARG;
+    std::vector<int64_t> fin;
+    for (int32_t s = 0; max_steps <= 0 || s < max_steps; s++) {
+        if (b->eng.active_ids().empty()) break;
+        const int rc = b->run.step(fin);
+        if (rc) {
+            b->err = b->eng.err;
+            return rc;
         }
     }
     return HMHD_OK;
 }
 
-int hmhd_synchronize(hmhd_t* h) {
-    if (!h) return HMHD_ERR_ARG;
-    return pass(h, h->eng.sync());
+int hmhd_batch_status(const hmhd_batch_t* b, int64_t id, double* t, int32_t* steps, int32_t* state) {
+    if (!b) return HMHD_ERR_ARG;
+    auto it = b->run.info.find(id);
+    if (it == b->run.info.end()) return HMHD_ERR_ARG;
+    if (t) *t = it->second.t;
+    if (steps) *steps = it->second.steps;
+    if (state) *state = it->second.state;
+    return HMHD_OK;
+}
+
+int hmhd_batch_dims(const hmhd_batch_t* b, int64_t id, int32_t* ngc, int32_t* mx, int32_t* my) {
+    if (!b) return HMHD_ERR_ARG;
+    const hmhd::InstHost* in = b->eng.find(id);
+    if (!in) return HMHD_ERR_ARG;
+    if (ngc) *ngc = in->ngc;
+    if (mx) *mx = in->mx;
+    if (my) *my = in->my;
+    return HMHD_OK;
+}
+
+int hmhd_batch_download(hmhd_batch_t* b, int64_t id, double* U, double* V, int32_t* scheme) {
+    if (!b) return HMHD_ERR_ARG;
+    const int rc = b->eng.download(id, U, V, scheme);
+    if (rc) b->err = b->eng.err;
+    return rc;
+}
+
+int hmhd_batch_profile(hmhd_batch_t* b, int32_t on) {
+    if (!b) return HMHD_ERR_ARG;
+    b->eng.prof_on = on != 0;
+    return HMHD_OK;
+}
+
+int hmhd_batch_profile_read(hmhd_batch_t* b, double* ms, int64_t* launches, int32_t reset) {
+    if (!b) return HMHD_ERR_ARG;
+    return profile_read(b->eng, ms, launches, reset);
+}
+
+// ------------------------------------------------------------------ //
+// server
+
+uint64_t hmhd_solve_footprint(const hmhd_config* cfg) {
+    std::string e;
+    if (!cfg || hmhd::Engine::check_config(*cfg, e) != HMHD_OK) return 0;
+    return hmhd::Engine::footprint(*cfg);
+}
+
+int hmhd_server_create(uint64_t hbm_budget_bytes, int32_t max_active, hmhd_server_t** out) {
+    if (!out || hbm_budget_bytes == 0)
+        return fail(g_create_err, HMHD_ERR_ARG, "hmhd_server_create: null handle or zero budget");
+    hmhd_server_t* s = new hmhd_server_handle;
+    s->budget = hbm_budget_bytes;
+    s->max_active = max_active;
+    s->worker = std::thread([s] { s->loop(); });
+    *out = s;
+    return HMHD_OK;
+}
+
+void hmhd_server_destroy(hmhd_server_t* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    delete s;
+}
+
+const char* hmhd_server_last_error(const hmhd_server_t* s) { return s ? s->err.c_str() : g_create_err.c_str(); }
+
+int hmhd_server_submit(hmhd_server_t* s, const hmhd_config* cfg, const double* U, const double* V, double tmax,
+                       int32_t nstepmax, int64_t* req) {
+    if (!s) return HMHD_ERR_ARG;
+    std::string e;
+    if (!cfg || !U || !req) e = "hmhd_server_submit: null argument";
+    else (void)hmhd::Engine::check_config(*cfg, e);
+    if (!e.empty()) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        return fail(s->err, HMHD_ERR_ARG, e);
+    }
+    hmhd_server_handle::Req r;
+    r.cfg = *cfg;
+    const int ngc = cfg->R + 2;
+    const size_t np = (size_t)(cfg->nx + 2 * ngc) * (cfg->ny + 2 * ngc);
+    r.U.assign(U, U + 8 * np);
+    if (V) r.V.assign(V, V + 10 * np);
+    r.tmax = tmax;
+    r.nstepmax = nstepmax;
+    r.bytes = hmhd::Engine::footprint(*cfg);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r.id = s->next_req++;
+        r.res.state = HMHD_REQ_QUEUED;
+        r.res.submit_s = s->now();
+        *req = r.id;
+        s->queue.push_back(r.id);
+        s->reqs.emplace(r.id, std::move(r));
+        s->st.submitted += 1;
+    }
+    s->cv_work.notify_all();
+    return HMHD_OK;
+}
+
+int hmhd_server_poll(hmhd_server_t* s, int64_t req, hmhd_result* res) {
+    if (!s) return HMHD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req);
+    if (it == s->reqs.end()) return fail(s->err, HMHD_ERR_ARG, "unknown request");
+    if (res) *res = it->second.res;
+    return HMHD_OK;
+}
+
+int hmhd_server_wait(hmhd_server_t* s, int64_t req, double timeout_s, hmhd_result* res, double* U, double* V,
+                     int32_t* scheme) {
+    if (!s) return HMHD_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req);
+    if (it == s->reqs.end()) return fail(s->err, HMHD_ERR_ARG, "unknown request");
+    hmhd_server_handle::Req& r = it->second;  // stays valid: only this call erases it
+    auto ready = [&] { return r.done; };
+    if (timeout_s > 0) {
+        if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
+            return fail(s->err, HMHD_ERR_TIMEOUT, "request not finished within the timeout");
+    } else {
+        s->cv_done.wait(lk, ready);
+    }
+    if (res) *res = r.res;
+    if (U && !r.Uout.empty()) std::memcpy(U, r.Uout.data(), sizeof(double) * r.Uout.size());
+    if (V && !r.Vout.empty()) std::memcpy(V, r.Vout.data(), sizeof(double) * r.Vout.size());
+    if (scheme && !r.Sout.empty()) std::memcpy(scheme, r.Sout.data(), sizeof(int32_t) * r.Sout.size());
+    s->reqs.erase(req);
+    return HMHD_OK;
+}
+
+int hmhd_server_stats_read(hmhd_server_t* s, hmhd_server_stats* out) {
+    if (!s || !out) return HMHD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    out->busy_s = s->busy_s;
+    out->elapsed_s = s->now();
+    return HMHD_OK;
 }
 
 }  // extern "C"

@@ -11,10 +11,10 @@
 //   FLB     the two FLICD-corrected components [2][my][mx]: By on x-faces,
 //           Bx on y-faces (src/FLICDdivB.f90:33-58).
 //   MOOD state on the padded grid, one byte per cell: scheme, rc
-//           (recompute), rcF (recomputeFLICD), inadm (inadmissible face
-//           states), low (detect verdict), rec (cells to re-reconstruct in
-//           the inner loop), faceRR[4] (radius each face was last
-//           reconstructed with).
+//           (recompute), rcF (recomputeFLICD), low (detect verdict), rec
+//           (cells to re-reconstruct in the inner loop), and per face side
+//           (N, S, E, W planes): faceRR[4] (radius the side was last
+//           reconstructed with), inadm[4] (the side's states failed CAD/PAD).
 //   ctl     per-instance flags read back by the host between MOOD passes.
 #pragma once
 
@@ -48,7 +48,11 @@ constexpr int kPvSet = gp_pv_off(kMaxR + 1);  // 596 doubles
 
 struct Inst {
     int32_t nx, ny, ngc, mx, my, R, bc, hydro;
-    int32_t gpset, active, pad0, pad1;
+    int32_t gpset, active;
+    int32_t cell_off, cell_cnt;  // CFL blocks of this instance
+    int32_t rot;                 // U[rot] holds U^n; stage buffers rotate from it
+    int32_t v0;                  // the next CFL reads the uploaded primitives V0
+    int32_t pad0, pad1;
     double dx, dy, mindxdy, maxdxdy;
     double gamma, small_p, small_rho, cfl;
     double dtdx, dtdy;
@@ -59,7 +63,7 @@ struct Inst {
     uint8_t* scheme;
     uint8_t* rc;
     uint8_t* rcF;
-    uint8_t* inadm;
+    uint8_t* inadm;   // [4][my][mx]
     uint8_t* low;
     uint8_t* rec;
     uint8_t* faceRR;  // [4][my][mx]
@@ -73,20 +77,22 @@ struct Item {
     int16_t tx, ty;
 };
 
-// tile sizes
-constexpr int kFaceTX = 16, kFaceTY = 16;  // face kernel (LDS-staged)
-constexpr int kCellT = 16;                 // update / mask kernels
+// tile sizes: every per-cell kernel works on the same 16 x 16 tiles of the
+// padded grid, so one active-tile list serves a whole MOOD iteration
+constexpr int kCellT = 16;
+constexpr int kFaceTX = kCellT, kFaceTY = kCellT;
 
 struct LaunchSet {
     const Inst* insts;
-    const Item* face;
-    int nface;
-    const Item* upd;
-    int nupd;
-    const Item* pad;
+    const Item* pad;   // every tile of the padded grids
     int npad;
+    const Item* band;  // tiles touching a guard band (periodic sync)
+    int nband;
     const Item* cell;  // interior tiles for CFL
     int ncell;
+    Item* act;         // active tiles of the next MOOD iteration
+    int32_t* act_count;
+    unsigned long long* cflpart;  // two partial maxima per CFL block
 };
 
 }  // namespace hmhd

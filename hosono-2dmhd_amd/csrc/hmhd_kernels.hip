@@ -15,12 +15,13 @@
 //   k_sync_*   periodic MOOD image sync (src/boundaryConditions.f90:189-254)
 //
 // The first pass of a stage runs k_face in FULL mode: every face at the top
-// radius, admissibility checked on the way.  Later passes (and the rare
-// first-pass redo when a reconstructed state was inadmissible) run the
-// inner admissibility loop with k_recon_check / k_lower_inner and then
-// k_face in FLUX mode, which recomputes only the faces the reference's
-// getFluxes_ would recompute, from the radius each side was last
-// reconstructed with.
+// radius, admissibility checked on the way.  Every later iteration (a new
+// MOOD pass, or another round of the inner admissibility loop when a
+// reconstructed state was inadmissible) runs k_lower_inner and k_face on the
+// active tiles only: the flagged cells are reconstructed and checked, and
+// the faces the reference's getFluxes_ would recompute are recomputed, each
+// side from the radius it was last reconstructed with.  k_tile_scan lists
+// the tiles the next iteration needs.
 #include <hip/hip_runtime.h>
 
 #include "hmhd_internal.hpp"
@@ -137,6 +138,20 @@ __device__ __forceinline__ bool pass_ready(const Inst& I) {
     return I.ctl[CTL_RUN] != 0 && I.ctl[CTL_ANY_INADM] == 0;
 }
 
+// SSP-RK3 buffer roles from the solve's rotation a = I.rot (U^n in U[a]):
+// stage 0 reads U[a] and writes U1 to U[a+1]; stage 1 reads U[a+1] and
+// writes U2 to U[a+2]; stage 2 reads U[a+2] and writes U^{n+1} to U[a+1];
+// stages 1 and 2 combine with U^n (src/timeStep.f90:161-169).
+struct StageBufs {
+    int in, out, base;
+};
+__device__ __forceinline__ StageBufs stage_bufs(int rot, int stage) {
+    const int a = rot, b = (rot + 1) % 3, c = (rot + 2) % 3;
+    if (stage == 0) return {a, b, a};
+    if (stage == 1) return {b, c, a};
+    return {c, b, a};
+}
+
 template <int DIR>
 __device__ __forceinline__ void accumulate_flux(double* acc, const Phys& ph, const double* uL, const double* uR,
                                                 double w, bool first, bool fog) {
@@ -168,45 +183,100 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
     I.scheme[idx] = R;
     I.rc[idx] = 1;
     I.rcF[idx] = 1;
-    I.inadm[idx] = 0;
 #pragma unroll
-    for (int f = 0; f < 4; f++) I.faceRR[f * plane + idx] = R;
+    for (int f = 0; f < 4; f++) {
+        I.faceRR[f * plane + idx] = R;
+        I.inadm[f * plane + idx] = 0;
+    }
+}
+
+// ------------------------------------------------------------------ //
+// One face after the first pass of a stage (src/reconstruct.f90:77-141 and
+// src/getFluxes.f90:26-88): a side whose cell is flagged for
+// reconstruction is rebuilt at the current EDP1 radius rr_now and its states
+// checked (CAD/PAD); any other side keeps the radius it was last
+// reconstructed with; the flux, when requested, uses rr_now points and
+// weights.
+template <int R, int DIR, class A>
+__device__ __forceinline__ void face_iter(const Phys& ph, int set, const A& aL, const A& aR, int fL, int fR,
+                                          bool recL, bool recR, int rr_now, int rrL_old, int rrR_old, bool flux,
+                                          double* acc, bool& badL, bool& badR) {
+    const int rrL = recL ? rr_now : rrL_old, rrR = recR ? rr_now : rrR_old;
+    const int nqL = (recL && rrL > 0) ? gp_nq(rrL) : 0;
+    const int nqR = (recR && rrR > 0) ? gp_nq(rrR) : 0;
+    const int nqF = flux ? (rr_now == 0 ? 1 : gp_nq(rr_now)) : 0;
+    int nq = nqL > nqR ? nqL : nqR;
+    nq = nqF > nq ? nqF : nq;
+    badL = badR = false;
+    for (int q = 0; q < nq; q++) {
+        double uL[NCONS], uR[NCONS];
+        reload_barrier();
+        if (q < nqL || q < nqF) side_value<R>(aL, set, rrL, fL, q, uL);
+        if (q < nqR || q < nqF) side_value<R>(aR, set, rrR, fR, q, uR);
+        if (q < nqL) badL |= !admissible(uL[DENS], cons2pres(ph, uL));
+        if (q < nqR) badR |= !admissible(uR[DENS], cons2pres(ph, uR));
+        if (q < nqF) accumulate_flux<DIR>(acc, ph, uL, uR, c_qw[set][rr_now][q], q == 0, rr_now == 0);
+    }
 }
 
 // ------------------------------------------------------------------ //
 // face kernel: each thread owns the x-face at i-1/2 and the y-face at j-1/2
-// of padded cell (i, j); the tile domain is [s-2, e+3]^2.
+// of padded cell (i, j), i.e. the east / north state of its west / south
+// neighbour and its own west / south state; cells of [s-2, e+3]^2 take part
+// (tiles of the padded grid, the same tiles every per-cell kernel uses).
+// FULL: the first pass of a stage -- every state of [s-2, e+2]^2 at radius
+// R, checked, and every flux of [s-1, e+2]^2.  Otherwise: one iteration of
+// the inner admissibility loop fused with the flux recomputation.
+// Inadmissibility is recorded per face side in four byte planes, so each
+// byte has a single writer.  Register budget: two waves per SIMD at R <= 2;
+// R = 3 keeps one (its stencils would spill at two).
+#ifndef HMHD_FACE_WAVES
+#define HMHD_FACE_WAVES 2
+#endif
 template <int R, bool FULL>
-__global__ __launch_bounds__(kFaceTX* kFaceTY) void k_face(const Inst* __restrict__ insts,
-                                                           const Item* __restrict__ items, int in_buf) {
+__global__ __launch_bounds__(kFaceTX* kFaceTY, R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(const Inst* __restrict__ insts,
+                                                                           const Item* __restrict__ items,
+                                                                           int stage) {
     constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1;
     constexpr int LX = TX + 2 * R + 1, LY = TY + 2 * R + 1, LP = LX * LY;
     __shared__ double sU[NCONS * LP];
 
     const Item it = items[blockIdx.x];
     const Inst& I = insts[it.inst];
+    if (I.R != R) return;
+    if (!FULL && !I.ctl[CTL_RUN]) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-    const int i0 = s - 2 + it.tx * TX, j0 = s - 2 + it.ty * TY;
+    const int i0 = it.tx * TX, j0 = it.ty * TY;
     const int i = i0 + lx, j = j0 + ly;
-    const bool in_dom = (i <= ex + 3) && (j <= ey + 3);
+    const bool in_dom = i >= s - 2 && i <= ex + 3 && j >= s - 2 && j <= ey + 3;
     const bool flux_cell = in_dom && i >= s - 1 && i <= ex + 2 && j >= s - 1 && j <= ey + 2;
-    const size_t idx = (size_t)j * mx + i;
+    const size_t idx = (size_t)j * mx + i, fplane = (size_t)mx * I.my;
+    // reconstruction range [s-2, e+2]^2 for the four cells around the two faces
+    const bool cellC = in_dom && i <= ex + 2 && j <= ey + 2;
+    const bool cellW = in_dom && i >= s - 1 && j <= ey + 2;
+    const bool cellS = in_dom && j >= s - 1 && i <= ex + 2;
 
-    bool do_x, do_y;
+    bool recC = false, recW = false, recS = false, fx = false, fy = false;
     if (FULL) {
-        do_x = in_dom && j <= ey + 2;
-        do_y = in_dom && i <= ex + 2;
+        recC = cellC;
+        recW = cellW;
+        recS = cellS;
+        fx = fy = flux_cell;
     } else {
-        if (!pass_ready(I)) return;
-        do_x = flux_cell && (I.rc[idx] || I.rc[idx - 1]);
-        do_y = flux_cell && (I.rc[idx] || I.rc[idx - mx]);
-        if (!__syncthreads_or(do_x || do_y)) return;
+        const uint8_t* flag = I.ctl[CTL_MODE] == MODE_NEWPASS ? I.rc : I.rec;
+        recC = cellC && flag[idx];
+        recW = cellW && flag[idx - 1];
+        recS = cellS && flag[idx - mx];
+        fx = flux_cell && (I.rc[idx] || I.rc[idx - 1]);
+        fy = flux_cell && (I.rc[idx] || I.rc[idx - mx]);
     }
+    const bool do_x = recC || recW || fx, do_y = recC || recS || fy;
+    if (!FULL && !__syncthreads_or(do_x || do_y)) return;
 
     // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
     {
-        const double* __restrict__ U = I.U[in_buf];
+        const double* __restrict__ U = I.U[stage_bufs(I.rot, stage).in];
         const size_t plane = (size_t)I.nx * I.ny;
         const int oi = i0 - H, oj = j0 - H;
         for (int c = threadIdx.x; c < LP; c += TX * TY) {
@@ -222,9 +292,10 @@ __global__ __launch_bounds__(kFaceTX* kFaceTY) void k_face(const Inst* __restric
 
     const Phys ph = phys_of(I);
     const int set = I.gpset;
-    const size_t fplane = (size_t)mx * I.my;
     const int cc = (ly + H) * LX + (lx + H);  // LDS offset of cell (i, j)
     const LdsAcc<LX, LP> aC{sU + cc}, aW{sU + cc - 1}, aS{sU + cc - LX};
+    uint8_t* const bad = I.inadm;  // planes [face][my][mx]
+    bool any_bad = false;
 
     if (FULL) {
         constexpr int NQ = gp_nq(R);
@@ -240,17 +311,12 @@ __global__ __launch_bounds__(kFaceTX* kFaceTY) void k_face(const Inst* __restric
                 gp_point<R>(aC, pv_ptr(set, R, FACE_W, q), uR);
                 badL |= !admissible(uL[DENS], cons2pres(ph, uL));
                 badR |= !admissible(uR[DENS], cons2pres(ph, uR));
-                if (flux_cell) accumulate_flux<0>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
+                if (fx) accumulate_flux<0>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
             }
-            if (badL && i >= s - 1) {
-                I.inadm[idx - 1] = 1;
-                I.ctl[CTL_ANY_INADM] = 1;
-            }
-            if (badR && i <= ex + 2) {
-                I.inadm[idx] = 1;
-                I.ctl[CTL_ANY_INADM] = 1;
-            }
-            if (flux_cell) {
+            if (recW) bad[FACE_E * fplane + idx - 1] = badL;
+            if (recC) bad[FACE_W * fplane + idx] = badR;
+            any_bad |= (recW && badL) || (recC && badR);
+            if (fx) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
                 I.FLB[idx] = acc[MAGY];
@@ -268,99 +334,65 @@ __global__ __launch_bounds__(kFaceTX* kFaceTY) void k_face(const Inst* __restric
                 gp_point<R>(aC, pv_ptr(set, R, FACE_S, q), uR);
                 badL |= !admissible(uL[DENS], cons2pres(ph, uL));
                 badR |= !admissible(uR[DENS], cons2pres(ph, uR));
-                if (flux_cell) accumulate_flux<1>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
+                if (fy) accumulate_flux<1>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
             }
-            if (badL && j >= s - 1) {
-                I.inadm[idx - mx] = 1;
-                I.ctl[CTL_ANY_INADM] = 1;
-            }
-            if (badR && j <= ey + 2) {
-                I.inadm[idx] = 1;
-                I.ctl[CTL_ANY_INADM] = 1;
-            }
-            if (flux_cell) {
+            if (recS) bad[FACE_N * fplane + idx - mx] = badL;
+            if (recC) bad[FACE_S * fplane + idx] = badR;
+            any_bad |= (recS && badL) || (recC && badR);
+            if (fy) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
                 I.FLB[fplane + idx] = acc[MAGX];
             }
         }
     } else {
+        const int sc = I.scheme[idx];
         if (do_x) {
-            const int rrL = I.faceRR[FACE_E * fplane + idx - 1], rrR = I.faceRR[FACE_W * fplane + idx];
-            const int sc = I.scheme[idx], sw = I.scheme[idx - 1];
-            const int rrF = sc < sw ? sc : sw;
-            const int nq = rrF == 0 ? 1 : gp_nq(rrF);
+            const int sw = I.scheme[idx - 1];
+            const int rr_now = sc < sw ? sc : sw;
+            bool badL, badR;
             double acc[NCONS];
-            for (int q = 0; q < nq; q++) {
-                double uL[NCONS], uR[NCONS];
-                reload_barrier();
-                side_value<R>(aW, set, rrL, FACE_E, q, uL);
-                side_value<R>(aC, set, rrR, FACE_W, q, uR);
-                accumulate_flux<0>(acc, ph, uL, uR, c_qw[set][rrF][q], q == 0, rrF == 0);
+            face_iter<R, 0>(ph, set, aW, aC, FACE_E, FACE_W, recW, recC, rr_now, I.faceRR[FACE_E * fplane + idx - 1],
+                            I.faceRR[FACE_W * fplane + idx], fx, acc, badL, badR);
+            if (recW) {
+                I.faceRR[FACE_E * fplane + idx - 1] = (uint8_t)rr_now;
+                bad[FACE_E * fplane + idx - 1] = badL;
             }
+            if (recC) {
+                I.faceRR[FACE_W * fplane + idx] = (uint8_t)rr_now;
+                bad[FACE_W * fplane + idx] = badR;
+            }
+            any_bad |= (recW && badL) || (recC && badR);
+            if (fx) {
 #pragma unroll
-            for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
-            I.FLB[idx] = acc[MAGY];
+                for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
+                I.FLB[idx] = acc[MAGY];
+            }
         }
         if (do_y) {
-            const int rrL = I.faceRR[FACE_N * fplane + idx - mx], rrR = I.faceRR[FACE_S * fplane + idx];
-            const int sc = I.scheme[idx], ss = I.scheme[idx - mx];
-            const int rrF = sc < ss ? sc : ss;
-            const int nq = rrF == 0 ? 1 : gp_nq(rrF);
+            const int ss = I.scheme[idx - mx];
+            const int rr_now = sc < ss ? sc : ss;
+            bool badL, badR;
             double acc[NCONS];
-            for (int q = 0; q < nq; q++) {
-                double uL[NCONS], uR[NCONS];
-                reload_barrier();
-                side_value<R>(aS, set, rrL, FACE_N, q, uL);
-                side_value<R>(aC, set, rrR, FACE_S, q, uR);
-                accumulate_flux<1>(acc, ph, uL, uR, c_qw[set][rrF][q], q == 0, rrF == 0);
+            face_iter<R, 1>(ph, set, aS, aC, FACE_N, FACE_S, recS, recC, rr_now, I.faceRR[FACE_N * fplane + idx - mx],
+                            I.faceRR[FACE_S * fplane + idx], fy, acc, badL, badR);
+            if (recS) {
+                I.faceRR[FACE_N * fplane + idx - mx] = (uint8_t)rr_now;
+                bad[FACE_N * fplane + idx - mx] = badL;
             }
+            if (recC) {
+                I.faceRR[FACE_S * fplane + idx] = (uint8_t)rr_now;
+                bad[FACE_S * fplane + idx] = badR;
+            }
+            any_bad |= (recS && badL) || (recC && badR);
+            if (fy) {
 #pragma unroll
-            for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
-            I.FLB[fplane + idx] = acc[MAGX];
+                for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
+                I.FLB[fplane + idx] = acc[MAGX];
+            }
         }
     }
-}
-
-// ------------------------------------------------------------------ //
-// inner admissibility loop, reconstruction half: every flagged cell of
-// [s-2, e+2]^2 reconstructs its four faces at the current EDP1 radius,
-// records that radius per face, and flags itself if a state is
-// inadmissible (src/reconstruct.f90:77-141).
-template <int R>
-__global__ __launch_bounds__(256) void k_recon_check(const Inst* __restrict__ insts, const Item* __restrict__ items,
-                                                     int in_buf) {
-    const Item it = items[blockIdx.x];
-    const Inst& I = insts[it.inst];
-    if (!I.ctl[CTL_RUN]) return;
-    const bool use_rc = I.ctl[CTL_MODE] == MODE_NEWPASS;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
-    const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    if (i < s - 2 || i > ex + 2 || j < s - 2 || j > ey + 2) return;
-    const size_t idx = (size_t)j * mx + i, fplane = (size_t)mx * I.my;
-    const bool flag = use_rc ? I.rc[idx] : I.rec[idx];
-    if (!flag) {
-        I.inadm[idx] = 0;
-        return;
-    }
-    const Phys ph = phys_of(I);
-    const GlobAcc a{I.U[in_buf], (size_t)I.nx * I.ny, i, j, s, I.nx, I.ny, I.bc};
-    const int sc = I.scheme[idx];
-    const int nb[4] = {I.scheme[idx + mx], I.scheme[idx - mx], I.scheme[idx + 1], I.scheme[idx - 1]};
-    bool bad = false;
-    for (int f = 0; f < 4; f++) {
-        const int rr = sc < nb[f] ? sc : nb[f];
-        I.faceRR[f * fplane + idx] = (uint8_t)rr;
-        if (rr == 0) continue;
-        for (int q = 0; q < gp_nq(rr); q++) {
-            double u[NCONS];
-            reload_barrier();
-            side_value<R>(a, I.gpset, rr, f, q, u);
-            bad |= !admissible(u[DENS], cons2pres(ph, u));
-        }
-    }
-    I.inadm[idx] = bad ? 1 : 0;
-    if (bad) I.ctl[CTL_ANY_INADM] = 1;
+    if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
 
 __device__ __forceinline__ uint8_t gather_or(const uint8_t* a, int i, int j, int mx, int my, int rad) {
@@ -378,6 +410,27 @@ __device__ __forceinline__ uint8_t gather_or(const uint8_t* a, int i, int j, int
     return r;
 }
 
+// a cell is inadmissible when any of its four face sides is
+__device__ __forceinline__ uint8_t inadm_cell(const Inst& I, size_t idx, size_t fplane) {
+    return I.inadm[idx] | I.inadm[fplane + idx] | I.inadm[2 * fplane + idx] | I.inadm[3 * fplane + idx];
+}
+
+__device__ __forceinline__ uint8_t gather_inadm(const Inst& I, int i, int j, int rad) {
+    const size_t fplane = (size_t)I.mx * I.my;
+    uint8_t r = 0;
+    for (int dj = -rad; dj <= rad; dj++) {
+        const int jj = j + dj;
+        if (jj < 0 || jj >= I.my) continue;
+        const int w = rad - (dj < 0 ? -dj : dj);
+        for (int di = -w; di <= w; di++) {
+            const int ii = i + di;
+            if (ii < 0 || ii >= I.mx) continue;
+            r |= inadm_cell(I, (size_t)jj * I.mx + ii, fplane);
+        }
+    }
+    return r;
+}
+
 // lowering of every inadmissible cell, as a gather (src/reconstruct.f90:146-171
 // with src/gridBlock.f90:194-235): rc |= cross, rcF |= diamond, rec = cross
 __global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const Item* __restrict__ items) {
@@ -387,9 +440,9 @@ __global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ in
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i >= I.mx || j >= I.my) return;
     const size_t idx = (size_t)j * I.mx + i;
-    const uint8_t cross = gather_or(I.inadm, i, j, I.mx, I.my, 1);
-    const uint8_t diam = gather_or(I.inadm, i, j, I.mx, I.my, 2);
-    if (I.inadm[idx]) {
+    const uint8_t cross = gather_inadm(I, i, j, 1);
+    const uint8_t diam = gather_inadm(I, i, j, 2);
+    if (inadm_cell(I, idx, (size_t)I.mx * I.my)) {
         const int sc = I.scheme[idx];
         I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
     }
@@ -455,7 +508,8 @@ __global__ __launch_bounds__(256) void k_sync_y(const Inst* __restrict__ insts, 
 }
 
 // ------------------------------------------------------------------ //
-// FLICD + FV update + detect + RK combination, cells [s, e+1]^2.
+// FLICD + FV update + detect + RK combination, cells [s, e+1]^2 of the
+// padded-grid tiles.
 // stage 0: out = Unew; stage 1: out = (3 base + Unew)/4;
 // stage 2: out = (base + 2 Unew)/3.
 
@@ -571,13 +625,13 @@ __device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const
 }
 
 __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Item* __restrict__ items,
-                                                int in_buf, int out_buf, int base_buf, int stage) {
+                                                int stage) {
     const Item it = items[blockIdx.x];
     const Inst& I = insts[it.inst];
     if (!pass_ready(I)) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    const int i = s + it.tx * kCellT + (threadIdx.x % kCellT), j = s + it.ty * kCellT + (threadIdx.x / kCellT);
-    if (i > ex + 1 || j > ey + 1) return;
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i < s || j < s || i > ex + 1 || j > ey + 1) return;
     const size_t idx = (size_t)j * mx + i, fp = (size_t)mx * I.my;
     const FaceRead fr{I.FR, fp, mx};
     const bool own = I.rcF[idx];
@@ -597,7 +651,8 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     const double cxe = I.rcF[idx + 1] ? fr.corr_x(i + 1, j) : I.FLB[idx + 1];
     const double cyn = I.rcF[idx + mx] ? fr.corr_y(i, j + 1) : I.FLB[fp + idx + mx];
     const double dtdx = I.dtdx, dtdy = I.dtdy;
-    const double* __restrict__ Uin = I.U[in_buf];
+    const StageBufs sb = stage_bufs(I.rot, stage);
+    const double* __restrict__ Uin = I.U[sb.in];
     double uc[NCONS], un[NCONS];
 #pragma unroll
     for (int v = 0; v < NCONS; v++) {
@@ -624,16 +679,16 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     }
     I.low[idx] = lower ? 1 : 0;
     if (lower) I.ctl[CTL_ANY_LOWER] = 1;
-    double* __restrict__ Uout = I.U[out_buf];
+    double* __restrict__ Uout = I.U[sb.out];
     if (stage == 0) {
 #pragma unroll
         for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = un[v];
     } else if (stage == 1) {
-        const double* __restrict__ Ub = I.U[base_buf];
+        const double* __restrict__ Ub = I.U[sb.base];
 #pragma unroll
         for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (3.0 * Ub[v * plane + ci] + un[v]) / 4.0;
     } else {
-        const double* __restrict__ Ub = I.U[base_buf];
+        const double* __restrict__ Ub = I.U[sb.base];
 #pragma unroll
         for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (Ub[v * plane + ci] + 2.0 * un[v]) / 3.0;
     }
@@ -643,11 +698,14 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
 // OP_BEGIN starts a stage; OP_NEXT decides, from the flags of the pass just
 // run, whether the instance continues its inner admissibility loop
 // (MODE_INNER), starts a new MOOD pass (MODE_NEWPASS) or leaves the loop.
-__global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int stage) {
+__global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int stage, int32_t* __restrict__ act_count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *act_count = 0;
     if (k >= ninst) return;
     int32_t* c = insts[k].ctl;
-    if (op == OP_BEGIN) {
+    if (!insts[k].active) {
+        c[CTL_RUN] = 0;
+    } else if (op == OP_BEGIN) {
         c[CTL_ANY_INADM] = 0;
         c[CTL_ANY_LOWER] = 0;
         c[CTL_RUN] = 1;
@@ -670,6 +728,27 @@ __global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int sta
     }
 }
 
+// Active tiles of the next MOOD iteration: every quantity a pass writes
+// (inner lowering, reconstruction, fluxes, update, detect bookkeeping) lies
+// within two cells of a cell flagged rc, rcF or inadm, so a tile takes part
+// when its footprint grown by two cells holds such a cell.  Appended in any
+// order: each tile's work is independent of the others'.
+__global__ __launch_bounds__(256) void k_tile_scan(const Inst* __restrict__ insts, const Item* __restrict__ items,
+                                                   Item* __restrict__ act, int32_t* __restrict__ act_count) {
+    const Item it = items[blockIdx.x];
+    const Inst& I = insts[it.inst];
+    const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
+    constexpr int W = kCellT + 4;
+    bool any = false;
+    for (int c = threadIdx.x; c < W * W; c += blockDim.x) {
+        const int i = x0 + c % W, j = y0 + c / W;
+        if (i < 0 || j < 0 || i >= I.mx || j >= I.my) continue;
+        const size_t idx = (size_t)j * I.mx + i;
+        any |= (I.rc[idx] | I.rcF[idx] | inadm_cell(I, idx, (size_t)I.mx * I.my)) != 0;
+    }
+    if (__syncthreads_or(any) && threadIdx.x == 0) act[atomicAdd(act_count, 1)] = it;
+}
+
 // ------------------------------------------------------------------ //
 // CFL: max over the interior of |v| + c_f from V = cons2prim(U), or from the
 // uploaded primitives before the first step (src/cfl.f90:28-48)
@@ -683,7 +762,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 }
 
 __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, const Item* __restrict__ items,
-                                             int buf, int use_v0) {
+                                             unsigned long long* __restrict__ part) {
     const Item it = items[blockIdx.x];
     const Inst& I = insts[it.inst];
     const int ki = it.tx * kCellT + (threadIdx.x % kCellT), kj = it.ty * kCellT + (threadIdx.x / kCellT);
@@ -691,7 +770,7 @@ __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, con
     if (ki < I.nx && kj < I.ny) {
         const size_t plane = (size_t)I.nx * I.ny, o = (size_t)kj * I.nx + ki;
         double d, vx, vy, p, b1, b2, b3;
-        if (use_v0) {
+        if (I.v0) {
             const double* V = I.V0;
             d = V[o];
             vx = V[plane + o];
@@ -704,7 +783,7 @@ __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, con
             const Phys ph = phys_of(I);
             double u[NCONS];
 #pragma unroll
-            for (int v = 0; v < NCONS; v++) u[v] = I.U[buf][v * plane + o];
+            for (int v = 0; v < NCONS; v++) u[v] = I.U[I.rot][v * plane + o];
             const Prim V = cons2prim(ph, u);
             d = V.d;
             vx = V.vx;
@@ -732,17 +811,56 @@ __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, con
     }
     bx = wave_max_u64(bx);
     by = wave_max_u64(by);
+    __shared__ unsigned long long wx[4], wy[4];
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&I.cflmax[0], bx);
-        atomicMax(&I.cflmax[1], by);
+        wx[threadIdx.x >> 6] = bx;
+        wy[threadIdx.x >> 6] = by;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) {
+            bx = wx[w] > bx ? wx[w] : bx;
+            by = wy[w] > by ? wy[w] : by;
+        }
+        part[2 * blockIdx.x] = bx;
+        part[2 * blockIdx.x + 1] = by;
     }
 }
 
-__global__ void k_cfl_reset(const Inst* __restrict__ insts, int ninst) {
+// second level: one block per instance over its block partials
+__global__ __launch_bounds__(256) void k_cfl_reduce(const Inst* __restrict__ insts,
+                                                    const unsigned long long* __restrict__ part) {
+    const Inst& I = insts[blockIdx.x];
+    unsigned long long bx = 0, by = 0;
+    for (int b = I.cell_off + threadIdx.x; b < I.cell_off + I.cell_cnt; b += blockDim.x) {
+        bx = part[2 * b] > bx ? part[2 * b] : bx;
+        by = part[2 * b + 1] > by ? part[2 * b + 1] : by;
+    }
+    bx = wave_max_u64(bx);
+    by = wave_max_u64(by);
+    __shared__ unsigned long long wx[4], wy[4];
+    if ((threadIdx.x & 63) == 0) {
+        wx[threadIdx.x >> 6] = bx;
+        wy[threadIdx.x >> 6] = by;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) {
+            bx = wx[w] > bx ? wx[w] : bx;
+            by = wy[w] > by ? wy[w] : by;
+        }
+        I.cflmax[0] = bx;
+        I.cflmax[1] = by;
+    }
+}
+
+// after a step: the active solves' U^{n+1} sits in U[rot+1]; the uploaded
+// primitives no longer describe the state
+__global__ void k_rotate(Inst* __restrict__ insts, int ninst) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < ninst) {
-        insts[k].cflmax[0] = 0ull;
-        insts[k].cflmax[1] = 0ull;
+    if (k < ninst && insts[k].active) {
+        insts[k].rot = (insts[k].rot + 1) % 3;
+        insts[k].v0 = 0;
     }
 }
 
@@ -759,9 +877,7 @@ __global__ void k_set_dt(Inst* __restrict__ insts, const double* __restrict__ dt
 // layout conversion between the reference's padded AoS U(8,mx,my) /
 // V(10,mx,my) and the device SoA interior arrays
 
-__global__ void k_unpack(const Inst* __restrict__ insts, int inst, const double* __restrict__ src, int nv,
-                         int to_v0, int buf) {
-    const Inst& I = insts[inst];
+__global__ void k_unpack(const Inst I, const double* __restrict__ src, int nv, int to_v0, int buf) {
     const size_t plane = (size_t)I.nx * I.ny;
     for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane; o += (size_t)gridDim.x * blockDim.x) {
         const int kj = (int)(o / I.nx), ki = (int)(o - (size_t)kj * I.nx);
@@ -778,9 +894,8 @@ __global__ void k_unpack(const Inst* __restrict__ insts, int inst, const double*
 }
 
 // padded AoS U, V = cons2prim(U) and scheme, guards by the index map
-__global__ void k_pack(const Inst* __restrict__ insts, int inst, int buf, double* __restrict__ Uo,
-                       double* __restrict__ Vo, int32_t* __restrict__ So) {
-    const Inst& I = insts[inst];
+__global__ void k_pack(const Inst I, int buf, double* __restrict__ Uo, double* __restrict__ Vo,
+                       int32_t* __restrict__ So) {
     const size_t n = (size_t)I.mx * I.my, plane = (size_t)I.nx * I.ny;
     const Phys ph = phys_of(I);
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
@@ -832,16 +947,21 @@ void launch_stage_init(const LaunchSet& L, hipStream_t st) {
 }
 
 void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, hipStream_t st) {
-    hipLaunchKernelGGL(k_ctl, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst, op, stage);
+    hipLaunchKernelGGL(k_ctl, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst, op, stage, L.act_count);
 }
 
-void launch_face(const LaunchSet& L, int R, bool full, int in_buf, hipStream_t st) {
-    const dim3 b(kFaceTX * kFaceTY), g = g1(L.nface);
+void launch_tile_scan(const LaunchSet& L, hipStream_t st) {
+    hipLaunchKernelGGL(k_tile_scan, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, L.act, L.act_count);
+}
+
+void launch_face(const LaunchSet& L, const Item* items, int n, int R, bool full, int stage, hipStream_t st) {
+    if (n <= 0) return;
+    const dim3 b(kFaceTX * kFaceTY), g = g1(n);
 #define HMHD_FACE(RR)                                                                     \
     if (full)                                                                             \
-        hipLaunchKernelGGL((k_face<RR, true>), g, b, 0, st, L.insts, L.face, in_buf);     \
+        hipLaunchKernelGGL((k_face<RR, true>), g, b, 0, st, L.insts, items, stage);       \
     else                                                                                  \
-        hipLaunchKernelGGL((k_face<RR, false>), g, b, 0, st, L.insts, L.face, in_buf);
+        hipLaunchKernelGGL((k_face<RR, false>), g, b, 0, st, L.insts, items, stage);
     if (R == 1) {
         HMHD_FACE(1)
     } else if (R == 2) {
@@ -852,50 +972,50 @@ void launch_face(const LaunchSet& L, int R, bool full, int in_buf, hipStream_t s
 #undef HMHD_FACE
 }
 
-void launch_recon_check(const LaunchSet& L, int R, int in_buf, hipStream_t st) {
-    if (R == 1)
-        hipLaunchKernelGGL(k_recon_check<1>, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, in_buf);
-    else if (R == 2)
-        hipLaunchKernelGGL(k_recon_check<2>, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, in_buf);
-    else
-        hipLaunchKernelGGL(k_recon_check<3>, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, in_buf);
+void launch_lower_inner(const LaunchSet& L, const Item* items, int n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_lower_inner, g1(n), dim3(256), 0, st, L.insts, items);
 }
 
-void launch_lower_inner(const LaunchSet& L, hipStream_t st) {
-    hipLaunchKernelGGL(k_lower_inner, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
+void launch_update(const LaunchSet& L, const Item* items, int n, int stage, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_update, g1(n), dim3(256), 0, st, L.insts, items, stage);
 }
 
-void launch_update(const LaunchSet& L, int in_buf, int out_buf, int base_buf, int stage, hipStream_t st) {
-    hipLaunchKernelGGL(k_update, g1(L.nupd), dim3(256), 0, st, L.insts, L.upd, in_buf, out_buf, base_buf, stage);
+void launch_lower_detect(const LaunchSet& L, const Item* items, int n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_lower_detect, g1(n), dim3(256), 0, st, L.insts, items);
 }
 
-void launch_lower_detect(const LaunchSet& L, bool periodic_any, hipStream_t st) {
-    hipLaunchKernelGGL(k_lower_detect, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
-    if (periodic_any) {
-        hipLaunchKernelGGL(k_sync_x, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
-        hipLaunchKernelGGL(k_sync_y, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
-    }
+void launch_sync(const LaunchSet& L, hipStream_t st) {
+    if (!L.nband) return;
+    hipLaunchKernelGGL(k_sync_x, g1(L.nband), dim3(256), 0, st, L.insts, L.band);
+    hipLaunchKernelGGL(k_sync_y, g1(L.nband), dim3(256), 0, st, L.insts, L.band);
 }
 
-void launch_cfl(const LaunchSet& L, int ninst, int buf, bool use_v0, hipStream_t st) {
-    hipLaunchKernelGGL(k_cfl_reset, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst);
-    hipLaunchKernelGGL(k_cfl, g1(L.ncell), dim3(256), 0, st, L.insts, L.cell, buf, use_v0 ? 1 : 0);
+void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st) {
+    if (L.ncell > 0) hipLaunchKernelGGL(k_cfl, g1(L.ncell), dim3(256), 0, st, L.insts, L.cell, L.cflpart);
+    hipLaunchKernelGGL(k_cfl_reduce, dim3(ninst), dim3(256), 0, st, L.insts, L.cflpart);
 }
 
 void launch_set_dt(Inst* insts, const double* dt, int ninst, hipStream_t st) {
     hipLaunchKernelGGL(k_set_dt, dim3((ninst + 63) / 64), dim3(64), 0, st, insts, dt, ninst);
 }
 
-void launch_unpack(const Inst* insts, int inst, const double* src, int nv, bool to_v0, int buf, size_t ncell,
-                   hipStream_t st) {
-    const int blocks = (int)((ncell + 255) / 256 < 4096 ? (ncell + 255) / 256 : 4096);
-    hipLaunchKernelGGL(k_unpack, dim3(blocks), dim3(256), 0, st, insts, inst, src, nv, to_v0 ? 1 : 0, buf);
+void launch_rotate(Inst* insts, int ninst, hipStream_t st) {
+    hipLaunchKernelGGL(k_rotate, dim3((ninst + 63) / 64), dim3(64), 0, st, insts, ninst);
 }
 
-void launch_pack(const Inst* insts, int inst, int buf, double* Uo, double* Vo, int32_t* So, size_t npad,
-                 hipStream_t st) {
+void launch_unpack(const Inst& I, const double* src, int nv, bool to_v0, int buf, hipStream_t st) {
+    const size_t ncell = (size_t)I.nx * I.ny;
+    const int blocks = (int)((ncell + 255) / 256 < 4096 ? (ncell + 255) / 256 : 4096);
+    hipLaunchKernelGGL(k_unpack, dim3(blocks), dim3(256), 0, st, I, src, nv, to_v0 ? 1 : 0, buf);
+}
+
+void launch_pack(const Inst& I, int buf, double* Uo, double* Vo, int32_t* So, hipStream_t st) {
+    const size_t npad = (size_t)I.mx * I.my;
     const int blocks = (int)((npad + 255) / 256 < 4096 ? (npad + 255) / 256 : 4096);
-    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(256), 0, st, insts, inst, buf, Uo, Vo, So);
+    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(256), 0, st, I, buf, Uo, Vo, So);
 }
 
 }  // namespace hmhd

@@ -1,7 +1,10 @@
-// hmhd_solver.cpp -- the host engine: device allocation, tile work lists,
-// and the MOOD-pass loop of each SSP-RK3 stage (src/timeStep.f90:45-171).
+// hmhd_solver.cpp -- the host engine: device memory of each solve, the
+// device-side instance table and tile work lists, and the MOOD-pass loop of
+// each SSP-RK3 stage (src/timeStep.f90:45-171) run for every active solve
+// at once.
 #include "hmhd_solver.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -12,7 +15,7 @@
 namespace hmhd {
 
 // ------------------------------------------------------------------ //
-// GP table registry (__constant__ sets are shared by every handle)
+// GP table registry (__constant__ sets are shared by every engine)
 namespace {
 struct GpSet {
     int refs = 0;
@@ -21,6 +24,9 @@ struct GpSet {
 };
 std::mutex g_gp_mu;
 GpSet g_gp[kMaxGpSets];
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 }  // namespace
 
 int gp_acquire(const double* pv, const double* qw, std::string& err) {
@@ -64,25 +70,25 @@ int Engine::hip_check(hipError_t e, const char* what) {
     return HMHD_ERR_DEVICE;
 }
 
+int Engine::ensure_stream() {
+    if (st_) return HMHD_OK;
+    return hip_check(hipStreamCreate(&st_), "hipStreamCreate");
+}
+
 Engine::~Engine() {
-    if (st) (void)hipStreamSynchronize(st);
-    for (auto& in : inst) {
-        for (void* p : in.allocs) (void)hipFree(p);
+    if (st_) (void)hipStreamSynchronize(st_);
+    for (auto& in : inst_) {
+        if (in.block) (void)hipFree(in.block);
         gp_release(in.gpset);
     }
-    for (Item* p : d_items) (void)hipFree(p);
-    if (d_insts) (void)hipFree(d_insts);
-    if (d_ctl) (void)hipFree(d_ctl);
-    if (d_dt) (void)hipFree(d_dt);
-    if (d_cfl) (void)hipFree(d_cfl);
-    if (d_stage) (void)hipFree(d_stage);
-    if (d_stage_i) (void)hipFree(d_stage_i);
-    if (h_ctl) (void)hipHostFree(h_ctl);
-    if (h_dt) (void)hipHostFree(h_dt);
-    if (h_cfl) (void)hipHostFree(h_cfl);
-    if (h_scheme) (void)hipHostFree(h_scheme);
-    for (auto e : ev_pool) (void)hipEventDestroy(e);
-    if (st) (void)hipStreamDestroy(st);
+    void* dev[] = {d_insts_, d_ctl_, d_cfl_, d_dt_, d_pad_, d_band_, d_cell_, d_act_, d_part_, d_stage_, d_stage_i_};
+    for (void* p : dev)
+        if (p) (void)hipFree(p);
+    void* host[] = {h_ctl_, h_cfl_, h_dt_, h_scheme_};
+    for (void* p : host)
+        if (p) (void)hipHostFree(p);
+    for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    if (st_) (void)hipStreamDestroy(st_);
 }
 
 int Engine::check_config(const hmhd_config& c, std::string& err) {
@@ -95,7 +101,8 @@ int Engine::check_config(const hmhd_config& c, std::string& err) {
         err = "grid_Nx and grid_Ny must be at least 2*(GP_maxRadius+2)";
         return HMHD_ERR_ARG;
     }
-    if ((long long)(c.nx + 2 * ngc) * (c.ny + 2 * ngc) > (1ll << 31)) {
+    if ((long long)(c.nx + 2 * ngc) * (c.ny + 2 * ngc) > (1ll << 31) ||
+        (c.nx + 2 * ngc + kCellT - 1) / kCellT > 32767 || (c.ny + 2 * ngc + kCellT - 1) / kCellT > 32767) {
         err = "grid too large for 32-bit cell indices";
         return HMHD_ERR_ARG;
     }
@@ -118,12 +125,22 @@ int Engine::check_config(const hmhd_config& c, std::string& err) {
     return HMHD_OK;
 }
 
-int Engine::add_instance(const hmhd_config& c) {
-    if (finalized) {
-        err = "engine already finalized";
-        return -1;
-    }
+// layout of one solve's device block: U[3], V0, FR, FLB, then the byte
+// planes scheme, rc, rcF, low, rec, faceRR[4], inadm[4]
+size_t Engine::footprint(const hmhd_config& c) {
+    const int ngc = c.R + 2;
+    const size_t nin = (size_t)c.nx * c.ny, npad = (size_t)(c.nx + 2 * ngc) * (c.ny + 2 * ngc);
+    size_t b = 0;
+    b += 4 * align_up(sizeof(double) * NCONS * nin);
+    b += align_up(sizeof(double) * 2 * NCONS * npad);
+    b += align_up(sizeof(double) * 2 * npad);
+    b += 5 * align_up(npad) + 2 * align_up(4 * npad);
+    return b;
+}
+
+int64_t Engine::add(const hmhd_config& c) {
     if (check_config(c, err) != HMHD_OK) return -1;
+    if (ensure_stream() != HMHD_OK) return -1;
     InstHost h;
     h.cfg = c;
     h.ngc = c.R + 2;
@@ -133,150 +150,238 @@ int Engine::add_instance(const hmhd_config& c) {
         err = "GP covariance matrix is not positive definite (src/linAlgQuadPrecision.f90:136)";
         return -1;
     }
-    inst.push_back(h);
-    return (int)inst.size() - 1;
+    h.gpset = gp_acquire(h.pv, h.qw, err);
+    if (h.gpset < 0) return -1;
+    h.bytes = footprint(c);
+    if (hip_check(hipMalloc(&h.block, h.bytes), "hipMalloc(solve)") ||
+        hip_check(hipMemsetAsync(h.block, 0, h.bytes, st_), "hipMemsetAsync(solve)")) {
+        if (h.block) (void)hipFree(h.block);
+        gp_release(h.gpset);
+        return -1;
+    }
+    const size_t nin = (size_t)c.nx * c.ny, npad = (size_t)h.mx * h.my;
+    char* p = static_cast<char*>(h.block);
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += align_up(bytes);
+        return q;
+    };
+    Inst& d = h.dev;
+    for (int b = 0; b < 3; b++) d.U[b] = reinterpret_cast<double*>(take(sizeof(double) * NCONS * nin));
+    d.V0 = reinterpret_cast<double*>(take(sizeof(double) * NCONS * nin));
+    d.FR = reinterpret_cast<double*>(take(sizeof(double) * 2 * NCONS * npad));
+    d.FLB = reinterpret_cast<double*>(take(sizeof(double) * 2 * npad));
+    d.scheme = reinterpret_cast<uint8_t*>(take(npad));
+    d.rc = reinterpret_cast<uint8_t*>(take(npad));
+    d.rcF = reinterpret_cast<uint8_t*>(take(npad));
+    d.low = reinterpret_cast<uint8_t*>(take(npad));
+    d.rec = reinterpret_cast<uint8_t*>(take(npad));
+    d.faceRR = reinterpret_cast<uint8_t*>(take(4 * npad));
+    d.inadm = reinterpret_cast<uint8_t*>(take(4 * npad));
+    d.nx = c.nx;
+    d.ny = c.ny;
+    d.ngc = h.ngc;
+    d.mx = h.mx;
+    d.my = h.my;
+    d.R = c.R;
+    d.bc = c.bc;
+    d.hydro = c.force_hydro ? 1 : 0;
+    d.gpset = h.gpset;
+    d.dx = (c.xend - c.xbeg) / c.nx;
+    d.dy = (c.yend - c.ybeg) / c.ny;
+    d.mindxdy = d.dy < d.dx ? d.dy : d.dx;
+    d.maxdxdy = d.dy > d.dx ? d.dy : d.dx;
+    d.gamma = c.gamma;
+    d.small_p = c.small_p;
+    d.small_rho = c.small_rho;
+    d.cfl = c.cfl;
+    d.rot = 0;
+    d.v0 = 0;
+    h.id = next_id_++;
+    bytes_used_ += h.bytes;
+    index_[h.id] = inst_.size();
+    inst_.push_back(std::move(h));
+    dirty_ = true;
+    return inst_.back().id;
 }
 
-static void tiles(std::vector<Item>& v, int k, int w, int h, int t) {
-    const int nt_x = (w + t - 1) / t, nt_y = (h + t - 1) / t;
+const InstHost* Engine::find(int64_t id) const {
+    auto it = index_.find(id);
+    return it == index_.end() ? nullptr : &inst_[it->second];
+}
+
+InstHost* Engine::find(int64_t id) {
+    auto it = index_.find(id);
+    return it == index_.end() ? nullptr : &inst_[it->second];
+}
+
+std::vector<int64_t> Engine::active_ids() const {
+    std::vector<int64_t> v;
+    for (auto& h : inst_)
+        if (h.active) v.push_back(h.id);
+    return v;
+}
+
+int Engine::remove(int64_t id) {
+    auto it = index_.find(id);
+    if (it == index_.end()) {
+        err = "unknown solve id";
+        return HMHD_ERR_ARG;
+    }
+    int rc = sync();
+    if (rc) return rc;
+    InstHost& h = inst_[it->second];
+    if (h.block) (void)hipFree(h.block);
+    gp_release(h.gpset);
+    bytes_used_ -= h.bytes;
+    inst_.erase(inst_.begin() + it->second);
+    index_.clear();
+    for (size_t k = 0; k < inst_.size(); k++) index_[inst_[k].id] = k;
+    dirty_ = true;
+    return HMHD_OK;
+}
+
+int Engine::set_active(int64_t id, bool on) {
+    InstHost* h = find(id);
+    if (!h) {
+        err = "unknown solve id";
+        return HMHD_ERR_ARG;
+    }
+    if (h->active != on) {
+        h->active = on;
+        dirty_ = true;
+    }
+    return HMHD_OK;
+}
+
+template <class T>
+int Engine::grow(T** p, size_t& cap, size_t need, bool host) {
+    if (need <= cap && *p) return HMHD_OK;
+    size_t n = std::max<size_t>(need, 2 * cap);
+    n = std::max<size_t>(n, 16);
+    if (*p) {
+        if (host)
+            (void)hipHostFree(*p);
+        else
+            (void)hipFree(*p);
+        *p = nullptr;
+    }
+    int rc = host ? hip_check(hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * n), "hipHostMalloc")
+                  : hip_check(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * n), "hipMalloc");
+    if (rc) return rc;
+    cap = n;
+    return HMHD_OK;
+}
+
+static void tiles(std::vector<Item>& v, int k, int w, int h) {
+    const int nt_x = (w + kCellT - 1) / kCellT, nt_y = (h + kCellT - 1) / kCellT;
     for (int ty = 0; ty < nt_y; ty++)
         for (int tx = 0; tx < nt_x; tx++) v.push_back(Item{k, (int16_t)tx, (int16_t)ty});
 }
 
-int Engine::finalize() {
-    if (finalized) return HMHD_OK;
-    if (inst.empty()) {
-        err = "no instances";
-        return HMHD_ERR_ARG;
-    }
+// rebuild the device instance table and the tile lists after membership
+// or activity changes (between steps only)
+int Engine::commit() {
+    if (!dirty_) return HMHD_OK;
     int rc;
-    if ((rc = hip_check(hipStreamCreate(&st), "hipStreamCreate"))) return rc;
-    const int n = (int)inst.size();
-    size_t max_pad = 0;
-    for (int k = 0; k < n; k++) {
-        InstHost& h = inst[k];
-        const hmhd_config& c = h.cfg;
-        h.gpset = gp_acquire(h.pv, h.qw, err);
-        if (h.gpset < 0) return HMHD_ERR_DEVICE;
-        const size_t nin = (size_t)c.nx * c.ny, npad = (size_t)h.mx * h.my;
-        max_pad = npad > max_pad ? npad : max_pad;
-        auto alloc = [&](size_t bytes, void** p) -> int {
-            int r = hip_check(hipMalloc(p, bytes), "hipMalloc");
-            if (r) return r;
-            h.allocs.push_back(*p);
-            return hip_check(hipMemsetAsync(*p, 0, bytes, st), "hipMemsetAsync");
-        };
-        Inst& d = h.dev;
-        d.nx = c.nx;
-        d.ny = c.ny;
-        d.ngc = h.ngc;
-        d.mx = h.mx;
-        d.my = h.my;
-        d.R = c.R;
-        d.bc = c.bc;
-        d.hydro = c.force_hydro ? 1 : 0;
-        d.gpset = h.gpset;
-        d.active = 1;
-        d.dx = (c.xend - c.xbeg) / c.nx;
-        d.dy = (c.yend - c.ybeg) / c.ny;
-        d.mindxdy = d.dy < d.dx ? d.dy : d.dx;
-        d.maxdxdy = d.dy > d.dx ? d.dy : d.dx;
-        d.gamma = c.gamma;
-        d.small_p = c.small_p;
-        d.small_rho = c.small_rho;
-        d.cfl = c.cfl;
-        for (int b = 0; b < 3; b++)
-            if ((rc = alloc(sizeof(double) * NCONS * nin, (void**)&d.U[b]))) return rc;
-        if ((rc = alloc(sizeof(double) * NCONS * nin, (void**)&d.V0))) return rc;
-        if ((rc = alloc(sizeof(double) * 2 * NCONS * npad, (void**)&d.FR))) return rc;
-        if ((rc = alloc(sizeof(double) * 2 * npad, (void**)&d.FLB))) return rc;
-        if ((rc = alloc(npad, (void**)&d.scheme))) return rc;
-        if ((rc = alloc(npad, (void**)&d.rc))) return rc;
-        if ((rc = alloc(npad, (void**)&d.rcF))) return rc;
-        if ((rc = alloc(npad, (void**)&d.inadm))) return rc;
-        if ((rc = alloc(npad, (void**)&d.low))) return rc;
-        if ((rc = alloc(npad, (void**)&d.rec))) return rc;
-        if ((rc = alloc(4 * npad, (void**)&d.faceRR))) return rc;
-        if (c.bc == HMHD_BC_PERIODIC) any_periodic = true;
+    if ((rc = ensure_stream())) return rc;
+    if ((rc = sync())) return rc;
+    const size_t n = inst_.size();
+    const size_t nctl = (size_t)CTL_N * n + 4;
+    if ((rc = grow(&d_insts_, cap_insts_, n, false))) return rc;
+    if ((rc = grow(&d_ctl_, cap_ctl_, nctl, false))) return rc;
+    if ((rc = grow(&h_ctl_, cap_hctl_, nctl, true))) return rc;
+    if ((rc = grow(&d_cfl_, cap_cfl_, 2 * n, false))) return rc;
+    if ((rc = grow(&h_cfl_, cap_hcfl_, 2 * n, true))) return rc;
+    if ((rc = grow(&d_dt_, cap_dt_, n, false))) return rc;
+    if ((rc = grow(&h_dt_, cap_hdt_, n, true))) return rc;
+    if ((rc = hip_check(hipMemsetAsync(d_ctl_, 0, sizeof(int32_t) * nctl, st_), "hipMemsetAsync(ctl)"))) return rc;
+
+    // tiles of the active solves, all 16 x 16 tiles of each padded grid:
+    // every tile, the tiles touching a guard band, the interior tiles (CFL)
+    std::vector<Item> pad, band, cell;
+    std::vector<Inst> table(n);
+    size_t max_pad = 0, sum_pad = 0;
+    for (int R = 0; R <= kMaxR; R++) has_R_[R] = false;
+    bool any_periodic = false;
+    for (size_t k = 0; k < n; k++) {
+        InstHost& h = inst_[k];
+        Inst d = h.dev;
+        d.ctl = d_ctl_ + CTL_N * k;
+        d.cflmax = d_cfl_ + 2 * k;
+        d.active = h.active ? 1 : 0;
+        d.v0 = h.v0_valid ? 1 : 0;
+        d.cell_off = (int32_t)cell.size();
+        d.cell_cnt = 0;
+        const size_t np = (size_t)h.mx * h.my;
+        max_pad = std::max(max_pad, np);
+        sum_pad += np;
+        if (h.active) {
+            has_R_[h.cfg.R] = true;
+            tiles(pad, (int)k, h.mx, h.my);
+            tiles(cell, (int)k, h.cfg.nx, h.cfg.ny);
+            d.cell_cnt = (int32_t)cell.size() - d.cell_off;
+            if (h.cfg.bc == HMHD_BC_PERIODIC) {
+                any_periodic = true;
+                const int ntx = (h.mx + kCellT - 1) / kCellT, nty = (h.my + kCellT - 1) / kCellT;
+                const int ex = h.ngc + h.cfg.nx - 1, ey = h.ngc + h.cfg.ny - 1;
+                for (int ty = 0; ty < nty; ty++)
+                    for (int tx = 0; tx < ntx; tx++) {
+                        const int x0 = tx * kCellT, x1 = x0 + kCellT - 1, y0 = ty * kCellT, y1 = y0 + kCellT - 1;
+                        if (x0 < h.ngc || x1 > ex || y0 < h.ngc || y1 > ey)
+                            band.push_back(Item{(int32_t)k, (int16_t)tx, (int16_t)ty});
+                    }
+            }
+        }
+        h.dev = d;
+        table[k] = d;
     }
-    if ((rc = hip_check(hipMalloc(&d_ctl, sizeof(int32_t) * CTL_N * n), "hipMalloc"))) return rc;
-    if ((rc = hip_check(hipMemsetAsync(d_ctl, 0, sizeof(int32_t) * CTL_N * n, st), "hipMemsetAsync"))) return rc;
-    if ((rc = hip_check(hipMalloc(&d_cfl, sizeof(unsigned long long) * 2 * n), "hipMalloc"))) return rc;
-    if ((rc = hip_check(hipMalloc(&d_dt, sizeof(double) * n), "hipMalloc"))) return rc;
-    if ((rc = hip_check(hipHostMalloc(&h_ctl, sizeof(int32_t) * CTL_N * n), "hipHostMalloc"))) return rc;
-    if ((rc = hip_check(hipHostMalloc(&h_dt, sizeof(double) * n), "hipHostMalloc"))) return rc;
-    if ((rc = hip_check(hipHostMalloc(&h_cfl, sizeof(unsigned long long) * 2 * n), "hipHostMalloc"))) return rc;
-    for (int k = 0; k < n; k++) {
-        inst[k].dev.ctl = d_ctl + CTL_N * k;
-        inst[k].dev.cflmax = d_cfl + 2 * k;
-    }
-    std::vector<Inst> hv(n);
-    for (int k = 0; k < n; k++) hv[k] = inst[k].dev;
-    if ((rc = hip_check(hipMalloc(&d_insts, sizeof(Inst) * n), "hipMalloc"))) return rc;
-    if ((rc = hip_check(hipMemcpy(d_insts, hv.data(), sizeof(Inst) * n, hipMemcpyHostToDevice), "hipMemcpy")))
+    if (n && (rc = hip_check(hipMemcpy(d_insts_, table.data(), sizeof(Inst) * n, hipMemcpyHostToDevice),
+                             "hipMemcpy(table)")))
         return rc;
-
-    // tile work lists: face domain [s-2, e+3]^2 per radius, update domain
-    // [s, e+1]^2, padded grid per radius, interior
-    std::vector<Item> upd, cell, pad_all;
-    std::vector<Item> faceR[kMaxR + 1], padR[kMaxR + 1];
-    for (int k = 0; k < n; k++) {
-        const hmhd_config& c = inst[k].cfg;
-        tiles(faceR[c.R], k, c.nx + 5, c.ny + 5, kFaceTX);
-        tiles(padR[c.R], k, inst[k].mx, inst[k].my, kCellT);
-        tiles(upd, k, c.nx + 1, c.ny + 1, kCellT);
-        tiles(cell, k, c.nx, c.ny, kCellT);
-    }
-    static_assert(kFaceTX == kFaceTY, "square face tiles assumed by the work list");
-    auto upload_items = [&](const std::vector<Item>& v, const Item** out, int* count) -> int {
-        *count = (int)v.size();
-        *out = nullptr;
-        if (v.empty()) return HMHD_OK;
-        Item* p = nullptr;
-        int r = hip_check(hipMalloc(&p, sizeof(Item) * v.size()), "hipMalloc");
-        if (r) return r;
-        d_items.push_back(p);
-        r = hip_check(hipMemcpy(p, v.data(), sizeof(Item) * v.size(), hipMemcpyHostToDevice), "hipMemcpy");
-        *out = p;
-        return r;
+    auto put = [&](const std::vector<Item>& v, Item** d, size_t& cap) -> int {
+        int r = grow(d, cap, v.size() + 1, false);
+        if (r || v.empty()) return r;
+        return hip_check(hipMemcpy(*d, v.data(), sizeof(Item) * v.size(), hipMemcpyHostToDevice), "hipMemcpy(items)");
     };
-    for (int R = 1; R <= kMaxR; R++) {
-        if ((rc = upload_items(faceR[R], &face_R[R], &nface_R[R]))) return rc;
-        if ((rc = upload_items(padR[R], &pad_R[R], &npad_R[R]))) return rc;
-        pad_all.insert(pad_all.end(), padR[R].begin(), padR[R].end());
-    }
-    L.insts = d_insts;
-    if ((rc = upload_items(upd, &L.upd, &L.nupd))) return rc;
-    if ((rc = upload_items(cell, &L.cell, &L.ncell))) return rc;
-    if ((rc = upload_items(pad_all, &L.pad, &L.npad))) return rc;
-
-    d_stage_n = max_pad * 10;
-    if ((rc = hip_check(hipMalloc(&d_stage, sizeof(double) * d_stage_n), "hipMalloc"))) return rc;
-    if ((rc = hip_check(hipMalloc(&d_stage_i, sizeof(int32_t) * max_pad), "hipMalloc"))) return rc;
-    h_scheme_n = 0;
-    for (auto& h : inst) h_scheme_n += (size_t)h.mx * h.my;
-    if ((rc = hip_check(hipHostMalloc(&h_scheme, h_scheme_n), "hipHostMalloc"))) return rc;
-
-    ctl_host.assign((size_t)CTL_N * n, 0);
-    stage_schemes.assign(n, {});
-    v0_valid.assign(n, false);
-    uploadedU.assign(n, {});
-    uploadedV.assign(n, {});
-    if ((rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"))) return rc;
-    finalized = true;
-    return HMHD_OK;
+    if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
+    if ((rc = put(band, &d_band_, cap_band_))) return rc;
+    if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
+    if ((rc = grow(&d_act_, cap_act_, pad.size() + 1, false))) return rc;
+    if ((rc = grow(&d_part_, cap_part_, 2 * cell.size() + 2, false))) return rc;
+    if ((rc = grow(&d_stage_, cap_stage_, 10 * max_pad, false))) return rc;
+    if ((rc = grow(&d_stage_i_, cap_stage_i_, max_pad, false))) return rc;
+    if ((rc = grow(&h_scheme_, cap_hscheme_, sum_pad, true))) return rc;
+    L.insts = d_insts_;
+    L.pad = d_pad_;
+    L.npad = (int)pad.size();
+    L.band = d_band_;
+    L.nband = any_periodic ? (int)band.size() : 0;
+    L.cell = d_cell_;
+    L.ncell = (int)cell.size();
+    L.act = d_act_;
+    L.act_count = d_ctl_ + (size_t)CTL_N * n;
+    L.cflpart = d_part_;
+    ctl_.assign((size_t)CTL_N * n, 0);
+    dirty_ = false;
+    return sync();
 }
 
-int Engine::set_gp(int k, const double* pv, const double* qw) {
-    InstHost& h = inst[k];
+int Engine::set_gp(int64_t id, const double* pv, const double* qw) {
+    InstHost* h = find(id);
+    if (!h) {
+        err = "unknown solve id";
+        return HMHD_ERR_ARG;
+    }
     int s = gp_acquire(pv, qw, err);
     if (s < 0) return HMHD_ERR_DEVICE;
-    gp_release(h.gpset);
-    h.gpset = s;
-    std::memcpy(h.pv, pv, sizeof h.pv);
-    std::memcpy(h.qw, qw, sizeof h.qw);
-    h.dev.gpset = s;
-    return hip_check(hipMemcpy(&d_insts[k].gpset, &s, sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy");
+    gp_release(h->gpset);
+    h->gpset = s;
+    h->dev.gpset = s;
+    std::memcpy(h->pv, pv, sizeof h->pv);
+    std::memcpy(h->qw, qw, sizeof h->qw);
+    dirty_ = true;
+    return HMHD_OK;
 }
 
 // ------------------------------------------------------------------ //
@@ -284,85 +389,79 @@ int Engine::set_gp(int k, const double* pv, const double* qw) {
 
 void Engine::prof_begin(int fam) {
     if (!prof_on) return;
-    if (ev_next + 2 > ev_pool.size()) {
+    if (ev_next_ + 2 > ev_pool_.size()) {
         for (int k = 0; k < 64; k++) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) return;
-            ev_pool.push_back(e);
+            ev_pool_.push_back(e);
         }
     }
-    ev_a = ev_pool[ev_next++];
-    ev_fam = fam;
-    (void)hipEventRecord(ev_a, st);
+    ev_a_ = ev_pool_[ev_next_++];
+    ev_fam_ = fam;
+    (void)hipEventRecord(ev_a_, st_);
 }
 
 void Engine::prof_end(int fam) {
-    if (!prof_on || ev_fam != fam || !ev_a) return;
-    hipEvent_t b = ev_pool[ev_next++];
-    (void)hipEventRecord(b, st);
-    ev_pending.push_back({fam, {ev_a, b}});
-    ev_a = nullptr;
-    ev_fam = -1;
+    if (!prof_on || ev_fam_ != fam || !ev_a_) return;
+    hipEvent_t b = ev_pool_[ev_next_++];
+    (void)hipEventRecord(b, st_);
+    ev_pending_.push_back({fam, {ev_a_, b}});
+    ev_a_ = nullptr;
+    ev_fam_ = -1;
 }
 
 void Engine::prof_collect() {
-    for (auto& p : ev_pending) {
+    for (auto& p : ev_pending_) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
             prof_ms[p.first] += ms;
             prof_n[p.first] += 1;
         }
     }
-    ev_pending.clear();
-    ev_next = 0;
+    ev_pending_.clear();
+    ev_next_ = 0;
 }
 
 // ------------------------------------------------------------------ //
 
 int Engine::read_ctl() {
-    const size_t n = inst.size();
-    int rc = hip_check(hipMemcpyAsync(h_ctl, d_ctl, sizeof(int32_t) * CTL_N * n, hipMemcpyDeviceToHost, st),
+    const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 1;
+    int rc = hip_check(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(int32_t) * nctl, hipMemcpyDeviceToHost, st_),
                        "hipMemcpyAsync(ctl)");
     if (rc) return rc;
-    if ((rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"))) return rc;
-    std::memcpy(ctl_host.data(), h_ctl, sizeof(int32_t) * CTL_N * n);
+    if ((rc = hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize"))) return rc;
+    std::memcpy(ctl_.data(), h_ctl_, sizeof(int32_t) * CTL_N * n);
+    act_n_ = h_ctl_[CTL_N * n];
     prof_collect();
+    if (prof_on) {
+        prof_n[PF_TILES] += act_n_;  // active tiles summed over MOOD iterations
+        prof_ms[PF_TILES] += 1.0;    // MOOD iterations (host round trips)
+    }
     return HMHD_OK;
 }
 
-void Engine::launch_face_groups(bool full, int in) {
-    for (int R = 1; R <= kMaxR; R++) {
-        if (!nface_R[R]) continue;
-        LaunchSet l = L;
-        l.face = face_R[R];
-        l.nface = nface_R[R];
-        launch_face(l, R, full, in, st);
-    }
+// templated kernels run once per GP radius present (tiles of other radii return)
+void Engine::launch_face_groups(const Item* items, int n, bool full, int stage) {
+    for (int R = 1; R <= kMaxR; R++)
+        if (has_R_[R]) launch_face(L, items, n, R, full, stage, st_);
 }
 
-void Engine::launch_recon_groups(int in) {
-    for (int R = 1; R <= kMaxR; R++) {
-        if (!npad_R[R]) continue;
-        LaunchSet l = L;
-        l.pad = pad_R[R];
-        l.npad = npad_R[R];
-        launch_recon_check(l, R, in, st);
-    }
-}
-
-// one RK stage: the MOOD loop of src/timeStep.f90:86-128 for every instance
-int Engine::run_stage(int s, int in, int out, int base) {
-    const int n = (int)inst.size();
-    launch_ctl(L, n, OP_BEGIN, s, st);
-    launch_stage_init(L, st);
+// one RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active solve
+int Engine::run_stage(int s) {
+    const int n = (int)inst_.size();
+    // first pass: every tile, every face at the top radius
+    launch_ctl(L, n, OP_BEGIN, s, st_);
+    launch_stage_init(L, st_);
     prof_begin(PF_FACE_FULL);
-    launch_face_groups(true, in);
+    launch_face_groups(L.pad, L.npad, true, s);
     prof_end(PF_FACE_FULL);
     prof_begin(PF_UPDATE);
-    launch_update(L, in, out, base, s, st);
+    launch_update(L, L.pad, L.npad, s, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
-    launch_lower_detect(L, any_periodic, st);
+    launch_lower_detect(L, L.pad, L.npad, st_);
+    launch_sync(L, st_);
+    launch_tile_scan(L, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
@@ -376,160 +475,188 @@ int Engine::run_stage(int s, int in, int out, int base) {
         }
         bool more = false;
         for (int k = 0; k < n; k++) {
-            const int32_t* c = &ctl_host[(size_t)CTL_N * k];
+            const int32_t* c = &ctl_[(size_t)CTL_N * k];
             if (c[CTL_RUN] && (c[CTL_ANY_INADM] || c[CTL_ANY_LOWER])) more = true;
         }
         if (!more) break;
-        launch_ctl(L, n, OP_NEXT, s, st);
+        // later passes and inner iterations: only the active tiles the
+        // previous iteration's scan listed
+        const int na = act_n_;
+        launch_ctl(L, n, OP_NEXT, s, st_);
         prof_begin(PF_RECON);
-        launch_lower_inner(L, st);
-        launch_recon_groups(in);
+        launch_lower_inner(L, L.act, na, st_);
         prof_end(PF_RECON);
         prof_begin(PF_FACE_FLUX);
-        launch_face_groups(false, in);
+        launch_face_groups(L.act, na, false, s);
         prof_end(PF_FACE_FLUX);
         prof_begin(PF_UPDATE);
-        launch_update(L, in, out, base, s, st);
+        launch_update(L, L.act, na, s, st_);
         prof_end(PF_UPDATE);
         prof_begin(PF_FLAGS);
-        launch_lower_detect(L, any_periodic, st);
+        launch_lower_detect(L, L.act, na, st_);
+        launch_sync(L, st_);
+        launch_tile_scan(L, st_);
         prof_end(PF_FLAGS);
         if ((rc = hip_check(hipGetLastError(), "kernel launch"))) return rc;
         if ((rc = read_ctl())) return rc;
     }
     if (record_schemes) {
         size_t off = 0;
-        for (auto& h : inst) {
+        for (auto& h : inst_) {
             const size_t np = (size_t)h.mx * h.my;
-            if ((rc = hip_check(hipMemcpyAsync(h_scheme + off, h.dev.scheme, np, hipMemcpyDeviceToHost, st),
+            if (h.active &&
+                (rc = hip_check(hipMemcpyAsync(h_scheme_ + off, h.dev.scheme, np, hipMemcpyDeviceToHost, st_),
                                 "hipMemcpyAsync(scheme)")))
                 return rc;
             off += np;
         }
-        if ((rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"))) return rc;
+        if ((rc = sync())) return rc;
         off = 0;
-        for (int k = 0; k < n; k++) {
-            const size_t np = (size_t)inst[k].mx * inst[k].my;
-            auto& dst = stage_schemes[k];
-            dst.resize(3 * np);
-            for (size_t p = 0; p < np; p++) dst[s * np + p] = h_scheme[off + p];
+        for (auto& h : inst_) {
+            const size_t np = (size_t)h.mx * h.my;
+            if (h.active) {
+                h.stage_schemes.resize(3 * np);
+                for (size_t p = 0; p < np; p++) h.stage_schemes[s * np + p] = h_scheme_[off + p];
+            }
             off += np;
         }
     }
     return HMHD_OK;
 }
 
-int Engine::advance(const double* dt_host) {
-    const int n = (int)inst.size();
-    std::memcpy(h_dt, dt_host, sizeof(double) * n);
-    int rc = hip_check(hipMemcpyAsync(d_dt, h_dt, sizeof(double) * n, hipMemcpyHostToDevice, st), "hipMemcpyAsync(dt)");
+int Engine::advance(const std::vector<double>& dt) {
+    int rc = commit();
     if (rc) return rc;
-    launch_set_dt(d_insts, d_dt, n, st);
-    const int a = cur, b = (cur + 1) % 3, c = (cur + 2) % 3;
+    const int n = (int)inst_.size();
+    size_t a = 0;
+    for (int k = 0; k < n; k++) h_dt_[k] = inst_[k].active ? (a < dt.size() ? dt[a++] : 0.0) : 0.0;
+    if (a != dt.size()) {
+        err = "advance: one dt per active solve expected";
+        return HMHD_ERR_ARG;
+    }
+    if ((rc = hip_check(hipMemcpyAsync(d_dt_, h_dt_, sizeof(double) * n, hipMemcpyHostToDevice, st_),
+                        "hipMemcpyAsync(dt)")))
+        return rc;
+    launch_set_dt(d_insts_, d_dt_, n, st_);
     // U1 = FE(U); U2 = (3U + FE(U1))/4; U^{n+1} = (U + 2 FE(U2))/3
-    if ((rc = run_stage(0, a, b, a))) return rc;
-    if ((rc = run_stage(1, b, c, a))) return rc;
-    if ((rc = run_stage(2, c, b, a))) return rc;
-    cur = b;
-    for (int k = 0; k < n; k++) v0_valid[k] = false;
-    return HMHD_OK;
+    for (int s = 0; s < 3; s++)
+        if ((rc = run_stage(s))) return rc;
+    launch_rotate(d_insts_, n, st_);
+    for (int k = 0; k < n; k++) {
+        InstHost& h = inst_[k];
+        if (!h.active) continue;
+        h.dev.rot = (h.dev.rot + 1) % 3;
+        h.v0_valid = false;
+        h.dev.v0 = 0;
+        for (int s = 0; s < 3; s++) {
+            h.passes[s] = ctl_[(size_t)CTL_N * k + CTL_PASSES + s];
+            h.inner[s] = ctl_[(size_t)CTL_N * k + CTL_INNER + s];
+        }
+    }
+    return hip_check(hipGetLastError(), "kernel launch");
 }
 
-int Engine::upload(int k, const double* U, const double* V) {
-    InstHost& h = inst[k];
-    const size_t np = (size_t)h.mx * h.my, nin = (size_t)h.cfg.nx * h.cfg.ny;
-    int rc = hip_check(hipMemcpy(d_stage, U, sizeof(double) * NCONS * np, hipMemcpyHostToDevice), "hipMemcpy(U)");
-    if (rc) return rc;
-    launch_unpack(d_insts, k, d_stage, NCONS, false, cur, nin, st);
-    if ((rc = hip_check(hipStreamSynchronize(st), "unpack U"))) return rc;
-    uploadedU[k].assign(U, U + NCONS * np);
-    if (V) {
-        if ((rc = hip_check(hipMemcpy(d_stage, V, sizeof(double) * 10 * np, hipMemcpyHostToDevice), "hipMemcpy(V)")))
-            return rc;
-        launch_unpack(d_insts, k, d_stage, 10, true, 0, nin, st);
-        if ((rc = hip_check(hipStreamSynchronize(st), "unpack V"))) return rc;
-        uploadedV[k].assign(V, V + 10 * np);
-        v0_valid[k] = true;
-    } else {
-        uploadedV[k].clear();
-        v0_valid[k] = false;
+int Engine::upload(int64_t id, const double* U, const double* V) {
+    InstHost* h = find(id);
+    if (!h) {
+        err = "unknown solve id";
+        return HMHD_ERR_ARG;
     }
+    int rc = commit();  // staging buffers sized for every solve
+    if (rc) return rc;
+    const size_t np = (size_t)h->mx * h->my;
+    if ((rc = hip_check(hipMemcpy(d_stage_, U, sizeof(double) * NCONS * np, hipMemcpyHostToDevice), "hipMemcpy(U)")))
+        return rc;
+    launch_unpack(h->dev, d_stage_, NCONS, false, h->dev.rot, st_);
+    if ((rc = sync())) return rc;
+    h->upU.assign(U, U + NCONS * np);
+    if (V) {
+        if ((rc = hip_check(hipMemcpy(d_stage_, V, sizeof(double) * 10 * np, hipMemcpyHostToDevice), "hipMemcpy(V)")))
+            return rc;
+        launch_unpack(h->dev, d_stage_, 10, true, 0, st_);
+        if ((rc = sync())) return rc;
+        h->upV.assign(V, V + 10 * np);
+    } else {
+        h->upV.clear();
+    }
+    h->v0_valid = V != nullptr;
+    dirty_ = true;  // the device table carries the v0 flag
     return HMHD_OK;
 }
 
 int Engine::cfl(std::vector<double>& dt_out) {
-    const int n = (int)inst.size();
-    // instances with uploaded primitives use them, the rest derive V from U
-    bool any_v0 = false, any_u = false;
-    for (int k = 0; k < n; k++) (v0_valid[k] ? any_v0 : any_u) = true;
-    prof_begin(PF_CFL);
-    if (any_v0 && any_u) {
-        err = "mixed first-step CFL sources in one batch";
-        return HMHD_ERR_ARG;
-    }
-    launch_cfl(L, n, cur, any_v0, st);
-    prof_end(PF_CFL);
-    int rc = hip_check(hipMemcpyAsync(h_cfl, d_cfl, sizeof(unsigned long long) * 2 * n, hipMemcpyDeviceToHost, st),
-                       "hipMemcpyAsync(cfl)");
+    int rc = commit();
     if (rc) return rc;
-    if ((rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"))) return rc;
+    const int n = (int)inst_.size();
+    prof_begin(PF_CFL);
+    launch_cfl(L, n, st_);
+    prof_end(PF_CFL);
+    if ((rc = hip_check(hipMemcpyAsync(h_cfl_, d_cfl_, sizeof(unsigned long long) * 2 * n, hipMemcpyDeviceToHost,
+                                       st_),
+                        "hipMemcpyAsync(cfl)")))
+        return rc;
+    if ((rc = sync())) return rc;
     prof_collect();
-    dt_out.resize(n);
+    dt_out.clear();
     for (int k = 0; k < n; k++) {
+        if (!inst_[k].active) continue;
         double xm, ym;
-        std::memcpy(&xm, &h_cfl[2 * k], 8);
-        std::memcpy(&ym, &h_cfl[2 * k + 1], 8);
-        const Inst& d = inst[k].dev;
+        std::memcpy(&xm, &h_cfl_[2 * k], 8);
+        std::memcpy(&ym, &h_cfl_[2 * k + 1], 8);
+        const Inst& d = inst_[k].dev;
         // dt = cfl * min(dx/xmax, dy/ymax) (src/cfl.f90:50); a NaN speed
         // gives a NaN dt so the driver's validTimeStep aborts
         const double a = d.dx / xm, b = d.dy / ym;
         double m = a < b ? a : b;
         if (std::isnan(a) || std::isnan(b)) m = std::nan("");
-        dt_out[k] = d.cfl * m;
+        dt_out.push_back(d.cfl * m);
     }
     return HMHD_OK;
 }
 
-int Engine::download(int k, double* U, double* V, int32_t* scheme) {
-    InstHost& h = inst[k];
-    const size_t np = (size_t)h.mx * h.my;
-    int rc;
-    if (v0_valid[k]) {
+int Engine::download(int64_t id, double* U, double* V, int32_t* scheme) {
+    InstHost* h = find(id);
+    if (!h) {
+        err = "unknown solve id";
+        return HMHD_ERR_ARG;
+    }
+    int rc = commit();
+    if (rc) return rc;
+    const size_t np = (size_t)h->mx * h->my;
+    if (h->v0_valid) {
         // no step since the upload: hand back what the caller uploaded
-        if (U) std::memcpy(U, uploadedU[k].data(), sizeof(double) * NCONS * np);
-        if (V) std::memcpy(V, uploadedV[k].data(), sizeof(double) * 10 * np);
+        if (U) std::memcpy(U, h->upU.data(), sizeof(double) * NCONS * np);
+        if (V) std::memcpy(V, h->upV.data(), sizeof(double) * 10 * np);
         U = V = nullptr;
     }
-    if (U || V) {
-        double* dU = d_stage;
-        double* dV = d_stage;
-        if (U && V) {
-            // two passes through the staging buffer
-            launch_pack(d_insts, k, cur, dU, nullptr, nullptr, np, st);
-            if ((rc = hip_check(hipMemcpyAsync(U, dU, sizeof(double) * NCONS * np, hipMemcpyDeviceToHost, st),
-                                "hipMemcpyAsync(U)")))
-                return rc;
-            if ((rc = hip_check(hipStreamSynchronize(st), "download U"))) return rc;
-            U = nullptr;
-        }
-        launch_pack(d_insts, k, cur, U ? dU : nullptr, V ? dV : nullptr, nullptr, np, st);
-        if (U &&
-            (rc = hip_check(hipMemcpyAsync(U, dU, sizeof(double) * NCONS * np, hipMemcpyDeviceToHost, st), "D2H U")))
+    const int buf = h->dev.rot;
+    if (U) {
+        launch_pack(h->dev, buf, d_stage_, nullptr, nullptr, st_);
+        if ((rc = hip_check(hipMemcpyAsync(U, d_stage_, sizeof(double) * NCONS * np, hipMemcpyDeviceToHost, st_),
+                            "D2H U")) ||
+            (rc = sync()))
             return rc;
-        if (V && (rc = hip_check(hipMemcpyAsync(V, dV, sizeof(double) * 10 * np, hipMemcpyDeviceToHost, st), "D2H V")))
+    }
+    if (V) {
+        launch_pack(h->dev, buf, nullptr, d_stage_, nullptr, st_);
+        if ((rc = hip_check(hipMemcpyAsync(V, d_stage_, sizeof(double) * 10 * np, hipMemcpyDeviceToHost, st_),
+                            "D2H V")) ||
+            (rc = sync()))
             return rc;
-        if ((rc = hip_check(hipStreamSynchronize(st), "download"))) return rc;
     }
     if (scheme) {
-        launch_pack(d_insts, k, cur, nullptr, nullptr, d_stage_i, np, st);
-        if ((rc = hip_check(hipMemcpyAsync(scheme, d_stage_i, sizeof(int32_t) * np, hipMemcpyDeviceToHost, st),
-                            "D2H scheme")))
+        launch_pack(h->dev, buf, nullptr, nullptr, d_stage_i_, st_);
+        if ((rc = hip_check(hipMemcpyAsync(scheme, d_stage_i_, sizeof(int32_t) * np, hipMemcpyDeviceToHost, st_),
+                            "D2H scheme")) ||
+            (rc = sync()))
             return rc;
-        if ((rc = hip_check(hipStreamSynchronize(st), "download scheme"))) return rc;
     }
     return HMHD_OK;
 }
 
-int Engine::sync() { return hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+int Engine::sync() {
+    if (!st_) return HMHD_OK;
+    return hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+}
 
 }  // namespace hmhd

@@ -1,12 +1,19 @@
 // hmhd_solver.hpp -- host-side engine shared by the single-run C ABI
-// (hmhd_*) and the batched / served forms: a set of independent instances
-// advanced together, one launch per kernel family covering every instance's
-// tiles.
+// (hmhd_*), the batched form (hmhd_batch_*) and the served form
+// (hmhd_server_*): a changing set of independent solves advanced together,
+// one launch per kernel family covering every active solve's tiles.
+//
+// Membership changes (add, remove, activate, deactivate) happen between
+// steps; the device-side instance table and tile lists are rebuilt lazily
+// before the next launch.  Every solve advances one step per engine step
+// with its own dt; all share the rotation of the three state buffers.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 
+#include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hmhd.h"
@@ -19,16 +26,23 @@ int gp_acquire(const double* pv, const double* qw, std::string& err);
 void gp_release(int set);
 
 struct InstHost {
-    hmhd_config cfg;
-    int ngc, mx, my;
+    int64_t id = -1;
+    hmhd_config cfg{};
+    int ngc = 0, mx = 0, my = 0;
     double pv[kPvSet];
     double qw[(kMaxR + 1) * 4];
     int gpset = -1;
-    Inst dev{};  // device pointers and constants, mirrored into the device array
-    std::vector<void*> allocs;
+    bool active = true;
+    Inst dev{};              // device pointers and constants, mirrored into the device table
+    void* block = nullptr;   // the one device allocation behind every array of this solve
+    size_t bytes = 0;
+    bool v0_valid = false;   // uploaded primitives feed the next CFL
+    std::vector<double> upU, upV;
+    std::vector<int32_t> stage_schemes;  // 3*mx*my when recording
+    int32_t passes[3] = {0, 0, 0}, inner[3] = {0, 0, 0};
 };
 
-enum ProfFamily : int { PF_FACE_FULL = 0, PF_FACE_FLUX, PF_UPDATE, PF_RECON, PF_FLAGS, PF_CFL, PF_N };
+enum ProfFamily : int { PF_FACE_FULL = 0, PF_FACE_FLUX, PF_UPDATE, PF_RECON, PF_FLAGS, PF_CFL, PF_TILES, PF_N };
 
 class Engine {
    public:
@@ -37,69 +51,95 @@ class Engine {
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
-    // validate cfg (errors as the reference's stop conditions), build GP tables
+    // validate cfg (errors as the reference's stop conditions)
     static int check_config(const hmhd_config& c, std::string& err);
-    int add_instance(const hmhd_config& c);  // returns index or -1 (see err)
-    int finalize();                           // allocate device arrays, item lists
+    // device bytes one solve of this configuration occupies
+    static size_t footprint(const hmhd_config& c);
 
-    int upload(int k, const double* U, const double* V);
-    int cfl(std::vector<double>& dt_out);     // dt of every instance
-    int advance(const double* dt_host);       // one SSP-RK3 step per instance
-    int download(int k, double* U, double* V, int32_t* scheme);
+    int64_t add(const hmhd_config& c);  // new solve (GP tables built, device memory zeroed); -1 on error
+    int remove(int64_t id);
+    int set_active(int64_t id, bool on);
+    int upload(int64_t id, const double* U, const double* V);
+    int download(int64_t id, double* U, double* V, int32_t* scheme);
+    int set_gp(int64_t id, const double* pv, const double* qw);
+    // dt of every active solve (cfl_computedt), in the order of active_ids()
+    int cfl(std::vector<double>& dt_out);
+    // one SSP-RK3 step of every active solve with its own dt (same order)
+    int advance(const std::vector<double>& dt);
     int sync();
-    int set_gp(int k, const double* pv, const double* qw);
+    const InstHost* find(int64_t id) const;
+    InstHost* find(int64_t id);
+    std::vector<int64_t> active_ids() const;
+    size_t bytes_in_use() const { return bytes_used_; }
+    size_t size() const { return inst_.size(); }
 
     std::string err;
-    std::vector<InstHost> inst;
-    hipStream_t st = nullptr;
-    int cur = 0;  // state buffer holding U^n (same rotation for all instances)
-    std::vector<int32_t> ctl_host;  // CTL_N words per instance (pinned mirror below)
     bool record_schemes = false;
-    std::vector<std::vector<int32_t>> stage_schemes;  // per instance, 3*mx*my
-    std::vector<bool> v0_valid;
-    std::vector<std::vector<double>> uploadedU, uploadedV;
-
-    // profiling
     bool prof_on = false;
     double prof_ms[PF_N] = {0};
     long long prof_n[PF_N] = {0};
 
    private:
     int hip_check(hipError_t e, const char* what);
+    int ensure_stream();
+    int commit();
     int read_ctl();
-    int run_stage(int s, int in, int out, int base);
+    int run_stage(int s);
     void prof_begin(int fam);
     void prof_end(int fam);
     void prof_collect();
-    void launch_face_groups(bool full, int in);
-    void launch_recon_groups(int in);
+    void launch_face_groups(const Item* items, int n, bool full, int in);
+    template <class T>
+    int grow(T** p, size_t& cap, size_t need, bool host);
 
-    Inst* d_insts = nullptr;
-    int32_t* d_ctl = nullptr;
-    int32_t* h_ctl = nullptr;  // pinned
-    double* d_dt = nullptr;
-    double* h_dt = nullptr;  // pinned
-    unsigned long long* d_cfl = nullptr;
-    unsigned long long* h_cfl = nullptr;  // pinned
-    uint8_t* h_scheme = nullptr;          // pinned staging for stage schemes
-    size_t h_scheme_n = 0;
-    double* d_stage = nullptr;            // staging for upload/download
-    size_t d_stage_n = 0;
-    int32_t* d_stage_i = nullptr;
-    std::vector<Item*> d_items;
+    std::vector<InstHost> inst_;  // dense order = device table order
+    std::unordered_map<int64_t, size_t> index_;
+    int64_t next_id_ = 0;
+    bool dirty_ = true;
+    size_t bytes_used_ = 0;
+    hipStream_t st_ = nullptr;
+
+    // device-side tables rebuilt by commit()
+    Inst* d_insts_ = nullptr;
+    size_t cap_insts_ = 0;
+    int32_t* d_ctl_ = nullptr;  // CTL_N words per solve, then the active-tile count
+    size_t cap_ctl_ = 0;
+    int32_t* h_ctl_ = nullptr;  // pinned mirror
+    size_t cap_hctl_ = 0;
+    unsigned long long* d_cfl_ = nullptr;  // two maxima per solve
+    size_t cap_cfl_ = 0;
+    unsigned long long* h_cfl_ = nullptr;
+    size_t cap_hcfl_ = 0;
+    double* d_dt_ = nullptr;
+    size_t cap_dt_ = 0;
+    double* h_dt_ = nullptr;
+    size_t cap_hdt_ = 0;
+    Item* d_pad_ = nullptr;
+    size_t cap_pad_ = 0;
+    Item* d_band_ = nullptr;
+    size_t cap_band_ = 0;
+    Item* d_cell_ = nullptr;
+    size_t cap_cell_ = 0;
+    Item* d_act_ = nullptr;
+    size_t cap_act_ = 0;
+    unsigned long long* d_part_ = nullptr;
+    size_t cap_part_ = 0;
+    uint8_t* h_scheme_ = nullptr;
+    size_t cap_hscheme_ = 0;
+    double* d_stage_ = nullptr;
+    size_t cap_stage_ = 0;
+    int32_t* d_stage_i_ = nullptr;
+    size_t cap_stage_i_ = 0;
     LaunchSet L{};
-    // face / recon items per radius
-    const Item* face_R[kMaxR + 1] = {nullptr};
-    int nface_R[kMaxR + 1] = {0};
-    const Item* pad_R[kMaxR + 1] = {nullptr};
-    int npad_R[kMaxR + 1] = {0};
-    bool any_periodic = false;
-    bool finalized = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
-    size_t ev_next = 0;
-    hipEvent_t ev_a = nullptr;
-    int ev_fam = -1;
+    int act_n_ = 0;
+    bool has_R_[kMaxR + 1] = {false};
+    std::vector<int32_t> ctl_;
+
+    std::vector<hipEvent_t> ev_pool_;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending_;
+    size_t ev_next_ = 0;
+    hipEvent_t ev_a_ = nullptr;
+    int ev_fam_ = -1;
 };
 
 }  // namespace hmhd

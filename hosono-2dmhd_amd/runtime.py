@@ -15,16 +15,22 @@ from __future__ import annotations
 import ctypes as C
 import dataclasses
 import math
+import os
 import pathlib
 
 import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = HERE / "libhmhd.so"
+# HMHD_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = pathlib.Path(os.environ.get("HMHD_LIB", HERE / "libhmhd.so"))
 
 BC_PERIODIC, BC_OUTFLOW = 0, 1
 RIEMANN_HLLC = 1
-PROFILE_FAMILIES = ("face_full", "face_flux", "update", "recon", "flags", "cfl")
+# "tiles": (MOOD iterations after each stage's first pass, active tiles they visited)
+PROFILE_FAMILIES = ("face_full", "face_flux", "update", "recon", "flags", "cfl", "tiles")
+RUN_ACTIVE, RUN_DONE, RUN_ABORTED = 0, 1, 2
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_ABORTED, REQ_FAILED = 0, 1, 2, 3, 4
+ERR_TIMEOUT = 4
 
 # every entry point include/hmhd.h declares
 EXPORTS = (
@@ -33,6 +39,11 @@ EXPORTS = (
     "hmhd_upload_state", "hmhd_cfl_dt", "hmhd_advance", "hmhd_download", "hmhd_stats",
     "hmhd_record_stage_schemes", "hmhd_stage_schemes", "hmhd_profile", "hmhd_profile_read",
     "hmhd_synchronize",
+    "hmhd_batch_create", "hmhd_batch_destroy", "hmhd_batch_last_error", "hmhd_batch_add", "hmhd_batch_run",
+    "hmhd_batch_status", "hmhd_batch_dims", "hmhd_batch_download", "hmhd_batch_profile",
+    "hmhd_batch_profile_read",
+    "hmhd_solve_footprint", "hmhd_server_create", "hmhd_server_destroy", "hmhd_server_last_error",
+    "hmhd_server_submit", "hmhd_server_poll", "hmhd_server_wait", "hmhd_server_stats_read",
 )
 
 
@@ -50,6 +61,18 @@ class _Config(C.Structure):
                 ("ell_over_delta", C.c_double), ("gamma", C.c_double), ("small_p", C.c_double),
                 ("small_rho", C.c_double), ("small_B", C.c_double), ("force_hydro", C.c_int32),
                 ("bc", C.c_int32), ("cfl", C.c_double), ("riemann", C.c_int32)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("state", C.c_int32), ("steps", C.c_int32), ("t", C.c_double), ("zone_updates", C.c_int64),
+                ("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("active", C.c_int32),
+                ("queued", C.c_int32), ("peak_active", C.c_int32), ("reserved", C.c_int32),
+                ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("engine_steps", C.c_int64),
+                ("zone_updates", C.c_int64), ("busy_s", C.c_double), ("elapsed_s", C.c_double)]
 
 
 @dataclasses.dataclass
@@ -110,6 +133,30 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.hmhd_profile.argtypes = [vp, C.c_int32]
     lib.hmhd_profile_read.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int32]
     lib.hmhd_synchronize.argtypes = [vp]
+    i64, i64p = C.c_int64, C.POINTER(C.c_int64)
+    lib.hmhd_batch_create.argtypes = [C.POINTER(vp)]
+    lib.hmhd_batch_destroy.argtypes = [vp]
+    lib.hmhd_batch_destroy.restype = None
+    lib.hmhd_batch_last_error.argtypes = [vp]
+    lib.hmhd_batch_last_error.restype = C.c_char_p
+    lib.hmhd_batch_add.argtypes = [vp, C.POINTER(_Config), dp, dp, C.c_double, C.c_int32, i64p]
+    lib.hmhd_batch_run.argtypes = [vp, C.c_int32]
+    lib.hmhd_batch_status.argtypes = [vp, i64, dp, ip, ip]
+    lib.hmhd_batch_dims.argtypes = [vp, i64, ip, ip, ip]
+    lib.hmhd_batch_download.argtypes = [vp, i64, dp, dp, ip]
+    lib.hmhd_batch_profile.argtypes = [vp, C.c_int32]
+    lib.hmhd_batch_profile_read.argtypes = [vp, dp, i64p, C.c_int32]
+    lib.hmhd_solve_footprint.argtypes = [C.POINTER(_Config)]
+    lib.hmhd_solve_footprint.restype = C.c_uint64
+    lib.hmhd_server_create.argtypes = [C.c_uint64, C.c_int32, C.POINTER(vp)]
+    lib.hmhd_server_destroy.argtypes = [vp]
+    lib.hmhd_server_destroy.restype = None
+    lib.hmhd_server_last_error.argtypes = [vp]
+    lib.hmhd_server_last_error.restype = C.c_char_p
+    lib.hmhd_server_submit.argtypes = [vp, C.POINTER(_Config), dp, dp, C.c_double, C.c_int32, i64p]
+    lib.hmhd_server_poll.argtypes = [vp, i64, C.POINTER(_Result)]
+    lib.hmhd_server_wait.argtypes = [vp, i64, C.c_double, C.POINTER(_Result), dp, dp, ip]
+    lib.hmhd_server_stats_read.argtypes = [vp, C.POINTER(_ServerStats)]
     if path is None:
         _LIB = lib
     return lib
@@ -244,6 +291,181 @@ class Solver:
 
     def synchronize(self):
         self._check(self.lib.hmhd_synchronize(self.h))
+
+
+def _padded(cfg: Config):
+    ngc = cfg.R + 2
+    return cfg.nx + 2 * ngc, cfg.ny + 2 * ngc
+
+
+def _state_args(cfg: Config, U, V):
+    mx, my = _padded(cfg)
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    if U.shape != (my, mx, 8):
+        raise ValueError(f"U must have shape {(my, mx, 8)}, got {U.shape}")
+    if V is not None:
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        if V.shape != (my, mx, 10):
+            raise ValueError(f"V must have shape {(my, mx, 10)}, got {V.shape}")
+    return U, V
+
+
+def footprint(cfg: Config) -> int:
+    """Device bytes one run of this configuration holds."""
+    c = cfg.to_c()
+    return int(load_library().hmhd_solve_footprint(C.byref(c)))
+
+
+class Batch:
+    """Many independent runs of the driver loop (src/mhd_driver.f90:76-95)
+    advanced together on one device; each keeps its own dt, t and end."""
+
+    def __init__(self):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.hmhd_batch_create(C.byref(h))
+        if rc != 0:
+            raise HmhdError(rc, self.lib.hmhd_batch_last_error(None).decode())
+        self.h = h
+        self.cfgs = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HmhdError(rc, self.lib.hmhd_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hmhd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, cfg: Config, U, V=None, tmax: float = 1.0, nstepmax: int = 0) -> int:
+        U, V = _state_args(cfg, U, V)
+        c = cfg.to_c()
+        i = C.c_int64()
+        self._check(self.lib.hmhd_batch_add(self.h, C.byref(c), _dp(U), _dp(V) if V is not None else None,
+                                            float(tmax), int(nstepmax), C.byref(i)))
+        self.cfgs[i.value] = cfg
+        return i.value
+
+    def run(self, max_steps: int = 0) -> None:
+        """Advance every unfinished run by up to max_steps steps (0: to the end)."""
+        self._check(self.lib.hmhd_batch_run(self.h, int(max_steps)))
+
+    def status(self, rid: int):
+        """(t, steps, state) with state one of RUN_ACTIVE, RUN_DONE, RUN_ABORTED."""
+        t, n, st = C.c_double(), C.c_int32(), C.c_int32()
+        self._check(self.lib.hmhd_batch_status(self.h, rid, C.byref(t), C.byref(n), C.byref(st)))
+        return t.value, n.value, st.value
+
+    def download(self, rid: int):
+        mx, my = _padded(self.cfgs[rid])
+        U = np.zeros((my, mx, 8))
+        V = np.zeros((my, mx, 10))
+        S = np.zeros((my, mx), dtype=np.int32)
+        self._check(self.lib.hmhd_batch_download(self.h, rid, _dp(U), _dp(V), _ip(S)))
+        return U, V, S
+
+    def profile(self, on: bool = True):
+        self._check(self.lib.hmhd_batch_profile(self.h, int(on)))
+
+    def profile_read(self, reset: bool = False) -> dict:
+        ms = np.zeros(len(PROFILE_FAMILIES))
+        n = np.zeros(len(PROFILE_FAMILIES), dtype=np.int64)
+        self._check(self.lib.hmhd_batch_profile_read(self.h, _dp(ms), n.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                     int(reset)))
+        return {f: (float(ms[k]), int(n[k])) for k, f in enumerate(PROFILE_FAMILIES)}
+
+
+@dataclasses.dataclass
+class Result:
+    state: int
+    steps: int
+    t: float
+    zone_updates: int
+    submit_s: float
+    admit_s: float
+    done_s: float
+    U: np.ndarray | None = None
+    V: np.ndarray | None = None
+    scheme: np.ndarray | None = None
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.submit_s
+
+
+class Server:
+    """Runs admitted by HBM budget and stepped together by a worker thread
+    in the library; submit/poll/wait are safe from any thread."""
+
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.hmhd_server_create(int(hbm_budget_bytes), int(max_active), C.byref(h))
+        if rc != 0:
+            raise HmhdError(rc, self.lib.hmhd_server_last_error(None).decode())
+        self.h = h
+        self.cfgs = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HmhdError(rc, self.lib.hmhd_server_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.hmhd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, cfg: Config, U, V=None, tmax: float = 1.0, nstepmax: int = 0) -> int:
+        U, V = _state_args(cfg, U, V)
+        c = cfg.to_c()
+        r = C.c_int64()
+        self._check(self.lib.hmhd_server_submit(self.h, C.byref(c), _dp(U), _dp(V) if V is not None else None,
+                                                float(tmax), int(nstepmax), C.byref(r)))
+        self.cfgs[r.value] = cfg
+        return r.value
+
+    def poll(self, req: int) -> Result:
+        res = _Result()
+        self._check(self.lib.hmhd_server_poll(self.h, req, C.byref(res)))
+        return Result(res.state, res.steps, res.t, res.zone_updates, res.submit_s, res.admit_s, res.done_s)
+
+    def wait(self, req: int, timeout_s: float = 0.0, arrays: bool = True) -> Result:
+        cfg = self.cfgs[req]
+        mx, my = _padded(cfg)
+        U = np.zeros((my, mx, 8)) if arrays else None
+        V = np.zeros((my, mx, 10)) if arrays else None
+        S = np.zeros((my, mx), dtype=np.int32) if arrays else None
+        res = _Result()
+        self._check(self.lib.hmhd_server_wait(self.h, req, float(timeout_s), C.byref(res),
+                                              _dp(U) if arrays else None, _dp(V) if arrays else None,
+                                              _ip(S) if arrays else None))
+        del self.cfgs[req]
+        return Result(res.state, res.steps, res.t, res.zone_updates, res.submit_s, res.admit_s, res.done_s,
+                      U, V, S)
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        self._check(self.lib.hmhd_server_stats_read(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_ if f != "reserved"}
 
 
 class TimeStepError(RuntimeError):

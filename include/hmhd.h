@@ -30,6 +30,7 @@ extern "C" {
 #define HMHD_ERR_ARG 1     /* invalid argument or configuration (the reference stops) */
 #define HMHD_ERR_DEVICE 2  /* HIP runtime error */
 #define HMHD_ERR_GP 3      /* GP covariance not positive definite (src/linAlgQuadPrecision.f90:136) */
+#define HMHD_ERR_TIMEOUT 4 /* hmhd_server_wait: the request did not finish in time */
 
 #define HMHD_BC_PERIODIC 0 /* sim_BC = 'periodic' */
 #define HMHD_BC_OUTFLOW 1  /* sim_BC = 'outflow'  */
@@ -112,12 +113,100 @@ int hmhd_stage_schemes(const hmhd_t* h, int32_t* out);
 /* Kernel timing with HIP events on the handle's stream.  When on != 0,
  * every launch is bracketed by events; hmhd_profile_read returns, per
  * kernel family, the total milliseconds and launch counts since the last
- * reset (order: face_full, face_flux, update, recon, flags, cfl). */
+ * reset (order: face_full, face_flux, update, recon, flags, cfl, tiles;
+ * "tiles" holds no time: ms[6] counts MOOD iterations after the first pass
+ * of each stage and launches[6] the active tiles they visited). */
 int hmhd_profile(hmhd_t* h, int32_t on);
 int hmhd_profile_read(hmhd_t* h, double* ms, int64_t* launches, int32_t reset);
 
 /* Wait for all work enqueued on the handle. */
 int hmhd_synchronize(hmhd_t* h);
+
+/* ------------------------------------------------------------------ *
+ * Batch: many independent runs of the driver loop of src/mhd_driver.f90:
+ * 76-95 (cfl_computedt, validTimeStep, timeStep_advanceSolution until
+ * sim_tmax or sim_nStep) advanced together, one kernel launch per family
+ * covering every run's tiles.  Runs may differ in size, R, boundary
+ * condition and physics; each keeps its own dt and t, and its results are
+ * bit-identical to the same run on its own handle.  The reference has no
+ * batched form: this replaces running its executable once per case.
+ * A batch is used by one host thread at a time.
+ */
+typedef struct hmhd_batch_handle hmhd_batch_t;
+
+#define HMHD_RUN_ACTIVE 0  /* still stepping */
+#define HMHD_RUN_DONE 1    /* reached tmax or nstepmax (src/mhd_driver.f90:76) */
+#define HMHD_RUN_ABORTED 2 /* dt negative or NaN (src/mhd_driver.f90:128-141) */
+
+int hmhd_batch_create(hmhd_batch_t** out);
+void hmhd_batch_destroy(hmhd_batch_t* b);
+const char* hmhd_batch_last_error(const hmhd_batch_t* b);
+/* Add a run from its initial state (layouts as hmhd_upload_state; V may be
+ * NULL); it stops at t = tmax or after nstepmax steps (nstepmax <= 0: no
+ * step limit).  *id names the run in the calls below. */
+int hmhd_batch_add(hmhd_batch_t* b, const hmhd_config* cfg, const double* U, const double* V, double tmax,
+                   int32_t nstepmax, int64_t* id);
+/* Advance every unfinished run by up to max_steps steps (max_steps <= 0:
+ * until all runs have finished). */
+int hmhd_batch_run(hmhd_batch_t* b, int32_t max_steps);
+/* t, steps taken and HMHD_RUN_* state of one run. */
+int hmhd_batch_status(const hmhd_batch_t* b, int64_t id, double* t, int32_t* steps, int32_t* state);
+int hmhd_batch_dims(const hmhd_batch_t* b, int64_t id, int32_t* ngc, int32_t* mx, int32_t* my);
+/* As hmhd_download, for one run of the batch. */
+int hmhd_batch_download(hmhd_batch_t* b, int64_t id, double* U, double* V, int32_t* scheme);
+/* As hmhd_profile / hmhd_profile_read, over every run of the batch. */
+int hmhd_batch_profile(hmhd_batch_t* b, int32_t on);
+int hmhd_batch_profile_read(hmhd_batch_t* b, double* ms, int64_t* launches, int32_t reset);
+
+/* ------------------------------------------------------------------ *
+ * Server: runs submitted at any time are admitted in arrival order while
+ * their device footprint fits the HBM budget (later, smaller requests are
+ * admitted past one that does not fit yet), stepped together with every
+ * other admitted run, and released when they finish.  A worker thread owns
+ * the device; the calls below are thread-safe.  Results are bit-identical
+ * to the same run on its own handle.  The reference has no served form.
+ */
+typedef struct hmhd_server_handle hmhd_server_t;
+
+#define HMHD_REQ_QUEUED 0
+#define HMHD_REQ_RUNNING 1
+#define HMHD_REQ_DONE 2    /* reached tmax or nstepmax */
+#define HMHD_REQ_ABORTED 3 /* dt negative or NaN */
+#define HMHD_REQ_FAILED 4  /* footprint above the budget, or a device error */
+
+typedef struct hmhd_result {
+    int32_t state;         /* HMHD_REQ_* */
+    int32_t steps;         /* steps taken */
+    double t;              /* simulation time reached */
+    int64_t zone_updates;  /* steps * nx * ny */
+    double submit_s, admit_s, done_s; /* seconds since the server started */
+} hmhd_result;
+
+typedef struct hmhd_server_stats {
+    int64_t submitted, completed;
+    int32_t active, queued, peak_active, reserved;
+    uint64_t bytes_in_use, peak_bytes; /* device bytes held by admitted runs */
+    int64_t engine_steps;              /* steps of the shared engine */
+    int64_t zone_updates;              /* of completed runs */
+    double busy_s, elapsed_s;          /* time stepping; time since start */
+} hmhd_server_stats;
+
+/* Device bytes one run of this configuration holds while admitted. */
+uint64_t hmhd_solve_footprint(const hmhd_config* cfg);
+/* max_active <= 0: no limit besides the budget. */
+int hmhd_server_create(uint64_t hbm_budget_bytes, int32_t max_active, hmhd_server_t** out);
+/* Stops the worker after its current step; unfinished runs are dropped. */
+void hmhd_server_destroy(hmhd_server_t* s);
+const char* hmhd_server_last_error(const hmhd_server_t* s);
+/* Queue a run (the state is copied; layouts as hmhd_upload_state). */
+int hmhd_server_submit(hmhd_server_t* s, const hmhd_config* cfg, const double* U, const double* V, double tmax,
+                       int32_t nstepmax, int64_t* req);
+int hmhd_server_poll(hmhd_server_t* s, int64_t req, hmhd_result* res);
+/* Block until the run has finished (timeout_s <= 0: no limit), copy its
+ * result and final U, V, scheme (any pointer may be NULL) and forget it. */
+int hmhd_server_wait(hmhd_server_t* s, int64_t req, double timeout_s, hmhd_result* res, double* U, double* V,
+                     int32_t* scheme);
+int hmhd_server_stats_read(hmhd_server_t* s, hmhd_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 
 def declared_entry_points():
     text = (ROOT / "include" / "hmhd.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(hmhd_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|uint64_t|const char\*)\s+(hmhd_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_entry_point():
