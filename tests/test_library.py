@@ -97,3 +97,37 @@ def test_initial_conditions_are_well_formed():
     U, V = ics.sedov(g, gamma=5.0 / 3.0, energy=1.0, p_bg=0.0, B=(0.0, 0.0, 0.0))
     js, is_ = g.interior()
     assert (U[js, is_, 4].sum() * g.dx * g.dy) == pytest.approx(1.0, rel=1e-14)
+
+
+def test_fortran_binding_compiles_and_links(tmp_path):
+    """The iso_c_binding module a Fortran driver uses compiles with the
+    image's Fortran compiler and a program using it links against the
+    library (nothing is run: that needs a device)."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang")
+    if fc is None:
+        pytest.skip("no Fortran compiler")
+    mod = ROOT / "hosono-2dmhd_amd" / "fortran" / "hmhd_c.f90"
+    prog = tmp_path / "use_it.f90"
+    prog.write_text(
+        "program use_it\n"
+        "  use hmhd_c\n"
+        "  use, intrinsic :: iso_c_binding\n"
+        "  type(hmhd_config) :: cfg\n"
+        "  type(c_ptr) :: h\n"
+        "  real(c_double) :: dt\n"
+        "  integer(c_int32_t) :: rc\n"
+        "  cfg%nx = 64; cfg%ny = 64; cfg%R = 2\n"
+        "  rc = hmhd_create(cfg, h)\n"
+        "  if (rc /= HMHD_OK) print *, hmhd_error_message(c_null_ptr)\n"
+        "  rc = hmhd_cfl_dt(h, dt)\n"
+        "  rc = hmhd_advance(h, dt)\n"
+        "  call hmhd_destroy(h)\n"
+        "end program use_it\n")
+    lib = rt.LIB_PATH
+    r = subprocess.run([fc, "-J", str(tmp_path), str(mod), str(prog), "-o", str(tmp_path / "use_it"),
+                        f"-L{lib.parent}", "-lhmhd", f"-Wl,-rpath,{lib.parent}"],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert (tmp_path / "use_it").exists()
