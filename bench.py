@@ -19,12 +19,19 @@ timed with HIP events on the solver's stream inside the timed region.
 `cpu_baseline` times the reference's own Fortran step (compiled from its
 sources by oracle/Makefile, OpenMP on the host cores) on a bounded sample of
 the same workload, in a child process started before the GPU is touched.
+
+Beside the plain form the line carries two more objects (--forms):
+`batched`, 256 seeded runs of mixed sizes advanced together in one engine,
+against the same runs launched one after another on their own handles; and
+`served`, Poisson arrivals of mixed requests into a server with an HBM
+budget, with p50/p99 submit-to-result latency and aggregate zone-updates/s.
 """
 from __future__ import annotations
 
 import argparse
 import importlib.util
 import json
+import math
 import os
 import pathlib
 import subprocess
@@ -140,31 +147,161 @@ def run_cpu_baseline(n: int, steps: int):
         return {"value": None, "unit": UNIT, "cores": None, "kind": None, "sample": "timed out"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the whole solve to t=0.5)")
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--tmax", type=float, default=0.5)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="steps in the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
-    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
-    args = ap.parse_args()
+# ---------------------------------------------------------------------- #
+# batched and served workloads
 
-    if args.cpu_child:
-        cpu_baseline_child(args.n, args.cpu_steps)
-        return 0
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this one-GPU build "
-              f"(a multi-GPU form would shard a solve by row bands with guard-row exchange, "
-              f"or place independent solves on separate devices)", file=sys.stderr)
-        return 2
+BATCH_SIDES = (64, 96, 128, 192, 256, 384)
+T_END_BATCH = {"ot": 0.5, "sedov": 0.05}
+T_END_SERVED = {"ot": 0.05, "blast": 0.01, "sedov": 0.005}
 
-    cpu = None if args.no_cpu else run_cpu_baseline(args.n, args.cpu_steps)
 
-    load_package()
-    import hosono_2dmhd_amd.runtime as rt
+def make_run(rt, kind: str, n: int, noise_seed: int | None):
+    """(config, U, V) of one run: the Orszag-Tang vortex above, the MHD blast
+    (rho = 1, p = 10 for r < 0.1 else 0.1, B = (1, 1, 0)/sqrt 2, outflow) or
+    the Sedov blast (E = 1 in the central 3x3 cells, rho = 1, p = 1e-5,
+    B = (0.5, 0, 0), outflow), optionally with 1% seeded uniform noise on
+    rho, v, p and B."""
+    import hosono_2dmhd_amd.ics as ics
+    gamma = 5.0 / 3.0
+    if kind == "ot":
+        g = ics.Grid(n, n, 0.0, 1.0, 0.0, 1.0, R=2)
+        U, V = ics.orszag_tang(g, gamma)
+        bc = rt.BC_PERIODIC
+    elif kind == "blast":
+        g = ics.Grid(n, n, -0.5, 0.5, -0.5, 0.5, R=2)
+        U, V = ics.mhd_blast(g, gamma)
+        bc = rt.BC_OUTFLOW
+    else:
+        g = ics.Grid(n, n, -0.5, 0.5, -0.5, 0.5, R=2)
+        U, V = ics.sedov(g, gamma, energy=1.0, p_bg=1e-5, B=(0.5, 0.0, 0.0))
+        bc = rt.BC_OUTFLOW
+    if noise_seed is not None:
+        U, V = ics.perturb(V, g, gamma, bc, noise_seed, 0.01)
+    cfg = rt.Config(n, n, g.xbeg, g.xend, g.ybeg, g.yend, R=2, ell_over_delta=12.0, gamma=gamma, bc=bc)
+    return cfg, U, V
+
+
+def solo_run(rt, cfg, U, V, tmax: float, nstepmax: int):
+    """One run on its own handle, the plain form; returns (seconds, steps)."""
+    with rt.Solver(cfg) as s:
+        s.upload(U, V)
+        s.synchronize()
+        t, steps = 0.0, 0
+        t0 = time.perf_counter()
+        while t < tmax and steps < nstepmax:
+            dt = rt.valid_time_step(s.cfl_computedt(), t, tmax)
+            s.advance_solution(dt)
+            t += dt
+            steps += 1
+        s.synchronize()
+        return time.perf_counter() - t0, steps
+
+
+def batched_form(rt, args) -> dict:
+    """256 runs (sides from BATCH_SIDES, Orszag-Tang or Sedov, 1% noise, all
+    seeded) advanced together, each for --batch-steps steps or to its end
+    time; then the same runs one after another on their own handles."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed)
+    runs = []
+    for k in range(args.batch_runs):
+        n = int(rng.choice(BATCH_SIDES))
+        kind = "ot" if rng.random() < 0.5 else "sedov"
+        runs.append((kind, n, make_run(rt, kind, n, args.seed * 100003 + k)))
+    with rt.Batch() as b:
+        ids = [b.add(cfg, U, V, T_END_BATCH[kind], args.batch_steps) for kind, n, (cfg, U, V) in runs]
+        t0 = time.perf_counter()
+        b.run(0)
+        batch_s = time.perf_counter() - t0
+        st = [b.status(i) for i in ids]
+        zu = sum(steps * n * n for (_, steps, _), (_, n, _) in zip(st, runs))
+        check = [b.download(i) for i in ids[:2]]
+    solo_s, solo_zu = 0.0, 0
+    for k, (kind, n, (cfg, U, V)) in enumerate(runs):
+        sec, steps = solo_run(rt, cfg, U, V, T_END_BATCH[kind], args.batch_steps)
+        solo_s += sec
+        solo_zu += steps * n * n
+    same = True
+    for k in range(len(check)):
+        kind, n, (cfg, U, V) = runs[k]
+        with rt.Solver(cfg) as s:
+            s.upload(U, V)
+            rt.run(s, T_END_BATCH[kind], args.batch_steps)
+            Us, Vs, _ = s.download()
+        same = same and bool(np.array_equal(Us, check[k][0]) and np.array_equal(Vs, check[k][1]))
+    sides = [n for _, n, _ in runs]
+    return {
+        "workload": f"{len(runs)} runs, sides {list(BATCH_SIDES)} and Orszag-Tang/Sedov drawn with seed "
+                    f"{args.seed}, 1% seeded noise, {args.batch_steps} steps each (or to t_end "
+                    f"{T_END_BATCH})",
+        "runs": len(runs), "cells": int(sum(n * n for n in sides)), "zone_updates": int(zu),
+        "value": zu / batch_s, "seconds": batch_s,
+        "per_instance_plain": {"value": solo_zu / solo_s, "seconds": solo_s,
+                               "note": "the same runs one after another, each on its own handle"},
+        "speedup_vs_per_instance": (zu / batch_s) / (solo_zu / solo_s),
+        "first_runs_bitwise_equal_to_solo": same,
+    }
+
+
+def served_form(rt, args) -> dict:
+    """Poisson arrivals of requests (sides log-uniform in [64, 1024],
+    Orszag-Tang / blast / Sedov, each to its end time in T_END_SERVED) into
+    a server with an HBM budget; latency is submit -> result."""
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 1)
+    reqs = []
+    t_arr = 0.0
+    for k in range(args.served_requests):
+        n = int(round(math.exp(rng.uniform(math.log(64.0), math.log(1024.0)))))
+        kind = ("ot", "blast", "sedov")[int(rng.integers(3))]
+        t_arr += float(rng.exponential(1.0 / args.served_rate))
+        cfg, U, V = make_run(rt, kind, n, None)
+        reqs.append((t_arr, kind, n, cfg, U, V))
+    budget = int(args.served_budget_gib * (1 << 30))
+    ids = [None] * len(reqs)
+    with rt.Server(budget) as srv:
+        def feeder():
+            t0 = time.perf_counter()
+            for k, (ta, kind, n, cfg, U, V) in enumerate(reqs):
+                delay = ta - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[k] = srv.submit(cfg, U, V, T_END_SERVED[kind], 0)
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        res = [srv.wait(i, timeout_s=600.0, arrays=False) for i in ids]
+        stats = srv.stats()
+    lat = np.array([r.latency_s for r in res])
+    zu = sum(r.zone_updates for r in res)
+    span = max(r.done_s for r in res) - min(r.submit_s for r in res)
+    solo_s, solo_zu = 0.0, 0
+    if args.served_plain:
+        for ta, kind, n, cfg, U, V in reqs:
+            sec, steps = solo_run(rt, cfg, U, V, T_END_SERVED[kind], 1_000_000)
+            solo_s += sec
+            solo_zu += steps * n * n
+    out = {
+        "workload": f"{len(reqs)} requests, Poisson rate {args.served_rate}/s, sides log-uniform in [64, 1024], "
+                    f"Orszag-Tang/blast/Sedov to t_end {T_END_SERVED}, seed {args.seed + 1}",
+        "hbm_budget_bytes": budget, "completed": int(sum(r.state == rt.REQ_DONE for r in res)),
+        "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()), "zone_updates": int(zu), "value": zu / span, "seconds": span,
+        "peak_active": stats["peak_active"], "peak_bytes": stats["peak_bytes"],
+        "engine_steps": stats["engine_steps"], "busy_fraction": stats["busy_s"] / span,
+    }
+    if args.served_plain:
+        out["per_instance_plain"] = {"value": solo_zu / solo_s, "seconds": solo_s,
+                                     "note": "the same requests one after another, each on its own handle"}
+        out["speedup_vs_per_instance"] = out["value"] / out["per_instance_plain"]["value"]
+    return out
+
+
+# ---------------------------------------------------------------------- #
+
+def plain_form(rt, args):
+    """The metric's workload: one Orszag-Tang solve, timed from t = 0."""
     g, U, V = workload(args.n)
     cfg = rt.Config(args.n, args.n, R=2, ell_over_delta=12.0)
 
@@ -175,9 +312,10 @@ def main() -> int:
             w.advance_solution(w.cfl_computedt())
         w.synchronize()
 
+    # timed region: HIP events bracket only the first-pass face kernel
     s = rt.Solver(cfg)
     s.upload(U, V)
-    s.profile(True)
+    s.profile(["face_full"])
     s.synchronize()
     s.profile_read(reset=True)
     tmax = args.tmax if args.steps == 0 else float("inf")
@@ -185,8 +323,20 @@ def main() -> int:
     t = 0.0
     steps = 0
     passes = 0
+    # every kernel family is timed over a window of steps from the middle
+    # of the solve (shocks formed, many MOOD passes); events around every
+    # launch slow those steps slightly
+    win0 = args.breakdown_at if args.steps == 0 else max(0, limit // 2 - args.breakdown_steps // 2)
+    win1 = win0 + args.breakdown_steps
+    snap0 = snap1 = None
     t0 = time.perf_counter()
     while t < tmax and steps < limit:
+        if steps == win0:
+            s.profile(True)
+            snap0 = s.profile_read()
+        elif steps == win1:
+            snap1 = s.profile_read()
+            s.profile(["face_full"])
         dt = rt.valid_time_step(s.cfl_computedt(), t, tmax)
         s.advance_solution(dt)
         t += dt
@@ -196,6 +346,10 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     prof = s.profile_read()
     s.close()
+    if snap1 is None:
+        snap1 = prof
+        win1 = steps
+    breakdown = {k: (snap1[k][0] - snap0[k][0], snap1[k][1] - snap0[k][1]) for k in snap1} if snap0 else {}
 
     zones = args.n * args.n
     value = steps * zones / elapsed
@@ -214,6 +368,7 @@ def main() -> int:
         "launches": n_full, "peak_note": "FP64 vector peak (equal to the FP64 MFMA peak on MI355X)",
         "hbm_achieved_gbs": zones * BYTES_FACE_KERNEL / avg_s / 1e9, "hbm_peak_gbs": PEAK_HBM_GBS,
     }
+    mood_iters, tiles = breakdown.pop("tiles", (0, 0))
     line = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
@@ -222,11 +377,60 @@ def main() -> int:
                    "ell_over_delta": 12.0, "riemann": "hllc", "cfl": 0.8, "t_end": t,
                    "form": "plain", "mood_passes_per_stage": passes / (3 * steps)},
         "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernel_ms": {k: v[0] for k, v in prof.items()},
-        "kernel_launches": {k: v[1] for k, v in prof.items()},
         "flops_per_zone_update": FLOP_ZONE_UPDATE_R2,
+        "kernel_breakdown": {"steps": [win0, win1],
+                             "ms": {k: v[0] for k, v in breakdown.items()},
+                             "launches": {k: v[1] for k, v in breakdown.items()},
+                             "mood_iterations": int(mood_iters), "active_tiles_visited": int(tiles)},
     }
+    return line
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the whole solve to t=0.5)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--tmax", type=float, default=0.5)
+    ap.add_argument("--breakdown-at", type=int, default=700, help="first step of the all-kernel timing window")
+    ap.add_argument("--breakdown-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--forms", default="plain,batched,served",
+                    help="comma list of plain, batched, served (the line's value is always the plain form's)")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--batch-runs", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--served-requests", type=int, default=48)
+    ap.add_argument("--served-rate", type=float, default=6.0, help="mean arrivals per second")
+    ap.add_argument("--served-budget-gib", type=float, default=8.0)
+    ap.add_argument("--served-plain", type=int, default=1, help="also run each request on its own handle")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
+    args = ap.parse_args()
+
+    if args.cpu_child:
+        cpu_baseline_child(args.n, args.cpu_steps)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this one-GPU build "
+              f"(a multi-GPU form would place independent solves on separate devices, or shard one "
+              f"solve into blocks with halo exchange)", file=sys.stderr)
+        return 2
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+
+    cpu = None if args.no_cpu else run_cpu_baseline(args.n, args.cpu_steps)
+
+    load_package()
+    import hosono_2dmhd_amd.runtime as rt
+    line = plain_form(rt, args)
+    line["cpu_baseline"] = cpu
+    if "batched" in forms:
+        line["batched"] = batched_form(rt, args)
+        print(f"batched: {line['batched']['value']:.4g} zone-updates/s", file=sys.stderr, flush=True)
+    if "served" in forms:
+        line["served"] = served_form(rt, args)
+        print(f"served: p50 {line['served']['p50_latency_s']:.3f} s", file=sys.stderr, flush=True)
     print(json.dumps(line))
     return 0
 

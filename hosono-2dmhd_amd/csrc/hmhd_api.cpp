@@ -369,7 +369,7 @@ int hmhd_stage_schemes(const hmhd_t* h, int32_t* out) {
 
 int hmhd_profile(hmhd_t* h, int32_t on) {
     if (!h) return HMHD_ERR_ARG;
-    h->eng.prof_on = on != 0;
+    h->eng.prof_mask = (uint32_t)on & ((1u << hmhd::PF_N) - 1u);
     return HMHD_OK;
 }
 
@@ -458,7 +458,7 @@ int hmhd_batch_download(hmhd_batch_t* b, int64_t id, double* U, double* V, int32
 
 int hmhd_batch_profile(hmhd_batch_t* b, int32_t on) {
     if (!b) return HMHD_ERR_ARG;
-    b->eng.prof_on = on != 0;
+    b->eng.prof_mask = (uint32_t)on & ((1u << hmhd::PF_N) - 1u);
     return HMHD_OK;
 }
 

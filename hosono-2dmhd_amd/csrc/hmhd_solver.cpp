@@ -388,7 +388,7 @@ int Engine::set_gp(int64_t id, const double* pv, const double* qw) {
 // profiling: events bracket each launch of a kernel family
 
 void Engine::prof_begin(int fam) {
-    if (!prof_on) return;
+    if (!(prof_mask >> fam & 1u)) return;
     if (ev_next_ + 2 > ev_pool_.size()) {
         for (int k = 0; k < 64; k++) {
             hipEvent_t e;
@@ -402,7 +402,7 @@ void Engine::prof_begin(int fam) {
 }
 
 void Engine::prof_end(int fam) {
-    if (!prof_on || ev_fam_ != fam || !ev_a_) return;
+    if (!(prof_mask >> fam & 1u) || ev_fam_ != fam || !ev_a_) return;
     hipEvent_t b = ev_pool_[ev_next_++];
     (void)hipEventRecord(b, st_);
     ev_pending_.push_back({fam, {ev_a_, b}});
@@ -433,7 +433,7 @@ int Engine::read_ctl() {
     std::memcpy(ctl_.data(), h_ctl_, sizeof(int32_t) * CTL_N * n);
     act_n_ = h_ctl_[CTL_N * n];
     prof_collect();
-    if (prof_on) {
+    if (prof_mask >> PF_TILES & 1u) {
         prof_n[PF_TILES] += act_n_;  // active tiles summed over MOOD iterations
         prof_ms[PF_TILES] += 1.0;    // MOOD iterations (host round trips)
     }

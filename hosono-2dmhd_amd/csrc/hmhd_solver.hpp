@@ -42,7 +42,16 @@ struct InstHost {
     int32_t passes[3] = {0, 0, 0}, inner[3] = {0, 0, 0};
 };
 
-enum ProfFamily : int { PF_FACE_FULL = 0, PF_FACE_FLUX, PF_UPDATE, PF_RECON, PF_FLAGS, PF_CFL, PF_TILES, PF_N };
+enum ProfFamily : int {
+    PF_FACE_FULL = 0,
+    PF_FACE_FLUX,
+    PF_UPDATE,
+    PF_RECON,
+    PF_FLAGS,
+    PF_CFL,
+    PF_TILES,  // counters, not times: MOOD iterations / active tiles
+    PF_N
+};
 
 class Engine {
    public:
@@ -75,7 +84,7 @@ class Engine {
 
     std::string err;
     bool record_schemes = false;
-    bool prof_on = false;
+    uint32_t prof_mask = 0;  // bit f: time kernel family f
     double prof_ms[PF_N] = {0};
     long long prof_n[PF_N] = {0};
 

@@ -26,7 +26,7 @@ LIB_PATH = pathlib.Path(os.environ.get("HMHD_LIB", HERE / "libhmhd.so"))
 
 BC_PERIODIC, BC_OUTFLOW = 0, 1
 RIEMANN_HLLC = 1
-# "tiles": (MOOD iterations after each stage's first pass, active tiles they visited)
+# "tiles" holds counts: (MOOD iterations, active tiles they visited)
 PROFILE_FAMILIES = ("face_full", "face_flux", "update", "recon", "flags", "cfl", "tiles")
 RUN_ACTIVE, RUN_DONE, RUN_ABORTED = 0, 1, 2
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_ABORTED, REQ_FAILED = 0, 1, 2, 3, 4
@@ -162,6 +162,14 @@ def load_library(path: pathlib.Path | str | None = None):
     return lib
 
 
+def _family_mask(families) -> int:
+    if families is True:
+        return (1 << len(PROFILE_FAMILIES)) - 1
+    if not families:
+        return 0
+    return sum(1 << PROFILE_FAMILIES.index(f) for f in families)
+
+
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -279,8 +287,10 @@ class Solver:
         self._check(self.lib.hmhd_stage_schemes(self.h, _ip(s)))
         return s
 
-    def profile(self, on: bool = True):
-        self._check(self.lib.hmhd_profile(self.h, int(on)))
+    def profile(self, families=True):
+        """Time kernel families with HIP events: True for all, False for
+        none, or an iterable of names from PROFILE_FAMILIES."""
+        self._check(self.lib.hmhd_profile(self.h, _family_mask(families)))
 
     def profile_read(self, reset: bool = False) -> dict:
         ms = np.zeros(len(PROFILE_FAMILIES))
@@ -374,8 +384,8 @@ class Batch:
         self._check(self.lib.hmhd_batch_download(self.h, rid, _dp(U), _dp(V), _ip(S)))
         return U, V, S
 
-    def profile(self, on: bool = True):
-        self._check(self.lib.hmhd_batch_profile(self.h, int(on)))
+    def profile(self, families=True):
+        self._check(self.lib.hmhd_batch_profile(self.h, _family_mask(families)))
 
     def profile_read(self, reset: bool = False) -> dict:
         ms = np.zeros(len(PROFILE_FAMILIES))

@@ -110,13 +110,16 @@ int hmhd_stats(const hmhd_t* h, int32_t* passes, int32_t* inner);
 int hmhd_record_stage_schemes(hmhd_t* h, int32_t on);
 int hmhd_stage_schemes(const hmhd_t* h, int32_t* out);
 
-/* Kernel timing with HIP events on the handle's stream.  When on != 0,
- * every launch is bracketed by events; hmhd_profile_read returns, per
- * kernel family, the total milliseconds and launch counts since the last
- * reset (order: face_full, face_flux, update, recon, flags, cfl, tiles;
- * "tiles" holds no time: ms[6] counts MOOD iterations after the first pass
- * of each stage and launches[6] the active tiles they visited). */
-int hmhd_profile(hmhd_t* h, int32_t on);
+/* Kernel timing with HIP events on the handle's stream.  `families` is a
+ * bit mask over the kernel families (bit f: family f in the order below;
+ * HMHD_PROFILE_ALL for every one, 0 to stop): each launch of a selected
+ * family is bracketed by events.  hmhd_profile_read returns, per family,
+ * the total milliseconds and launch counts since the last reset (order:
+ * face_full, face_flux, update, recon, flags, cfl, tiles; "tiles" holds
+ * counts, not times: ms[6] counts MOOD iterations (host round trips) and
+ * launches[6] the active tiles they visited). */
+#define HMHD_PROFILE_ALL 0x7f
+int hmhd_profile(hmhd_t* h, int32_t families);
 int hmhd_profile_read(hmhd_t* h, double* ms, int64_t* launches, int32_t reset);
 
 /* Wait for all work enqueued on the handle. */
@@ -155,7 +158,7 @@ int hmhd_batch_dims(const hmhd_batch_t* b, int64_t id, int32_t* ngc, int32_t* mx
 /* As hmhd_download, for one run of the batch. */
 int hmhd_batch_download(hmhd_batch_t* b, int64_t id, double* U, double* V, int32_t* scheme);
 /* As hmhd_profile / hmhd_profile_read, over every run of the batch. */
-int hmhd_batch_profile(hmhd_batch_t* b, int32_t on);
+int hmhd_batch_profile(hmhd_batch_t* b, int32_t families);
 int hmhd_batch_profile_read(hmhd_batch_t* b, double* ms, int64_t* launches, int32_t reset);
 
 /* ------------------------------------------------------------------ *
