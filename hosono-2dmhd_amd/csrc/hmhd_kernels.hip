@@ -220,10 +220,17 @@ __device__ __forceinline__ void face_iter(const Phys& ph, int set, const A& aL, 
 }
 
 // ------------------------------------------------------------------ //
-// face kernel: each thread owns the x-face at i-1/2 and the y-face at j-1/2
-// of padded cell (i, j), i.e. the east / north state of its west / south
-// neighbour and its own west / south state; cells of [s-2, e+3]^2 take part
-// (tiles of the padded grid, the same tiles every per-cell kernel uses).
+// face kernel, one block per 16 x 16 tile.  Padded cell c owns the x-face
+// at i-1/2 (the east state of its west neighbour and its own west state)
+// and the y-face at j-1/2 (north state of the south neighbour, own south
+// state).  In the later passes at R <= 2 the block has 2 x 256 threads and
+// thread (d, c) does face d of cell c: splitting the directions halves a
+// block's latency, which bounds those sparse passes.  The first pass keeps
+// 256 threads doing both faces (two blocks per CU overlap one's LDS fill
+// with the other's arithmetic), as does R = 3, whose registers for one
+// direction already fill a thread at two waves per SIMD.
+// Cells of [s-2, e+3]^2 take part (tiles of the padded grid, the same
+// tiles every per-cell kernel uses).
 // FULL: the first pass of a stage -- every state of [s-2, e+2]^2 at radius
 // R, checked, and every flux of [s-1, e+2]^2.  Otherwise: one iteration of
 // the inner admissibility loop fused with the flux recomputation.
@@ -233,11 +240,11 @@ __device__ __forceinline__ void face_iter(const Phys& ph, int set, const A& aL, 
 #ifndef HMHD_FACE_WAVES
 #define HMHD_FACE_WAVES 2
 #endif
+__host__ __device__ constexpr int face_threads(int R, bool full) { return (R >= 3 || full ? 1 : 2) * kFaceTX * kFaceTY; }
 template <int R, bool FULL>
-__global__ __launch_bounds__(kFaceTX* kFaceTY, R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(const Inst* __restrict__ insts,
-                                                                           const Item* __restrict__ items,
-                                                                           int stage) {
-    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1;
+__global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
+    const Inst* __restrict__ insts, const Item* __restrict__ items, int stage) {
+    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, FULL);
     constexpr int LX = TX + 2 * R + 1, LY = TY + 2 * R + 1, LP = LX * LY;
     __shared__ double sU[NCONS * LP];
 
@@ -246,7 +253,9 @@ __global__ __launch_bounds__(kFaceTX* kFaceTY, R >= 3 ? 1 : HMHD_FACE_WAVES) voi
     if (I.R != R) return;
     if (!FULL && !I.ctl[CTL_RUN]) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    // 0: x-faces, 1: y-faces, 2: both (wave-uniform)
+    const int dir = NT > TX * TY ? (int)(threadIdx.x / (TX * TY)) : 2;
+    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY;
     const int i0 = it.tx * TX, j0 = it.ty * TY;
     const int i = i0 + lx, j = j0 + ly;
     const bool in_dom = i >= s - 2 && i <= ex + 3 && j >= s - 2 && j <= ey + 3;
@@ -271,7 +280,7 @@ __global__ __launch_bounds__(kFaceTX* kFaceTY, R >= 3 ? 1 : HMHD_FACE_WAVES) voi
         fx = flux_cell && (I.rc[idx] || I.rc[idx - 1]);
         fy = flux_cell && (I.rc[idx] || I.rc[idx - mx]);
     }
-    const bool do_x = recC || recW || fx, do_y = recC || recS || fy;
+    const bool do_x = dir != 1 && (recC || recW || fx), do_y = dir != 0 && (recC || recS || fy);
     if (!FULL && !__syncthreads_or(do_x || do_y)) return;
 
     // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
@@ -279,7 +288,7 @@ __global__ __launch_bounds__(kFaceTX* kFaceTY, R >= 3 ? 1 : HMHD_FACE_WAVES) voi
         const double* __restrict__ U = I.U[stage_bufs(I.rot, stage).in];
         const size_t plane = (size_t)I.nx * I.ny;
         const int oi = i0 - H, oj = j0 - H;
-        for (int c = threadIdx.x; c < LP; c += TX * TY) {
+        for (int c = threadIdx.x; c < LP; c += NT) {
             const int cy = c / LX, cx = c - cy * LX;
             const int kx = map_axis(oi + cx, s, I.nx, I.bc), ky = map_axis(oj + cy, s, I.ny, I.bc);
             const double* src = U + (size_t)ky * I.nx + kx;
@@ -956,12 +965,12 @@ void launch_tile_scan(const LaunchSet& L, hipStream_t st) {
 
 void launch_face(const LaunchSet& L, const Item* items, int n, int R, bool full, int stage, hipStream_t st) {
     if (n <= 0) return;
-    const dim3 b(kFaceTX * kFaceTY), g = g1(n);
-#define HMHD_FACE(RR)                                                                     \
-    if (full)                                                                             \
-        hipLaunchKernelGGL((k_face<RR, true>), g, b, 0, st, L.insts, items, stage);       \
-    else                                                                                  \
-        hipLaunchKernelGGL((k_face<RR, false>), g, b, 0, st, L.insts, items, stage);
+    const dim3 g = g1(n);
+#define HMHD_FACE(RR)                                                                                        \
+    if (full)                                                                                                \
+        hipLaunchKernelGGL((k_face<RR, true>), g, dim3(face_threads(RR, true)), 0, st, L.insts, items, stage); \
+    else                                                                                                     \
+        hipLaunchKernelGGL((k_face<RR, false>), g, dim3(face_threads(RR, false)), 0, st, L.insts, items, stage);
     if (R == 1) {
         HMHD_FACE(1)
     } else if (R == 2) {
