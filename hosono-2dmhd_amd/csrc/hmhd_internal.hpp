@@ -82,6 +82,15 @@ struct Item {
 constexpr int kCellT = 16;
 constexpr int kFaceTX = kCellT, kFaceTY = kCellT;
 
+// a list of tiles: `n` items, or -- when `cnt` is set -- as many as the
+// device-side count says (the active list the previous MOOD iteration's
+// scan wrote, unknown to the host when it enqueues the next iteration)
+struct TileList {
+    const Item* items;
+    int n;
+    const int32_t* cnt;
+};
+
 struct LaunchSet {
     const Inst* insts;
     const Item* pad;   // every tile of the padded grids
@@ -90,8 +99,8 @@ struct LaunchSet {
     int nband;
     const Item* cell;  // interior tiles for CFL
     int ncell;
-    Item* act;         // active tiles of the next MOOD iteration
-    int32_t* act_count;
+    Item* act[2];        // active tiles, written by alternate MOOD iterations
+    int32_t* act_count;  // [2] their counts
     unsigned long long* cflpart;  // two partial maxima per CFL block
 };
 

@@ -241,14 +241,17 @@ __device__ __forceinline__ void face_iter(const Phys& ph, int set, const A& aL, 
 #define HMHD_FACE_WAVES 2
 #endif
 __host__ __device__ constexpr int face_threads(int R, bool full) { return (R >= 3 || full ? 1 : 2) * kFaceTX * kFaceTY; }
-template <int R, bool FULL>
-__global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
-    const Inst* __restrict__ insts, const Item* __restrict__ items, int stage) {
-    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, FULL);
-    constexpr int LX = TX + 2 * R + 1, LY = TY + 2 * R + 1, LP = LX * LY;
-    __shared__ double sU[NCONS * LP];
+template <int R>
+constexpr int face_lds_cells() {
+    return (kFaceTX + 2 * R + 1) * (kFaceTY + 2 * R + 1);
+}
 
-    const Item it = items[blockIdx.x];
+template <int R, bool FULL>
+__device__ __forceinline__ void face_tile(const Inst* __restrict__ insts, const Item it, int stage,
+                                          double* __restrict__ sU) {
+    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, FULL);
+    constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
+
     const Inst& I = insts[it.inst];
     if (I.R != R) return;
     if (!FULL && !I.ctl[CTL_RUN]) return;
@@ -404,6 +407,17 @@ __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
 
+template <int R, bool FULL>
+__global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
+    const Inst* __restrict__ insts, const TileList tl, int stage) {
+    __shared__ double sU[NCONS * face_lds_cells<R>()];
+    const int n = tl.cnt ? *tl.cnt : tl.n;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        if (b != (int)blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
+        face_tile<R, FULL>(insts, tl.items[b], stage, sU);
+    }
+}
+
 __device__ __forceinline__ uint8_t gather_or(const uint8_t* a, int i, int j, int mx, int my, int rad) {
     uint8_t r = 0;
     for (int dj = -rad; dj <= rad; dj++) {
@@ -442,8 +456,7 @@ __device__ __forceinline__ uint8_t gather_inadm(const Inst& I, int i, int j, int
 
 // lowering of every inadmissible cell, as a gather (src/reconstruct.f90:146-171
 // with src/gridBlock.f90:194-235): rc |= cross, rcF |= diamond, rec = cross
-__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const Item* __restrict__ items) {
-    const Item it = items[blockIdx.x];
+__device__ __forceinline__ void lower_inner_tile(const Inst* __restrict__ insts, const Item it) {
     const Inst& I = insts[it.inst];
     if (!I.ctl[CTL_RUN] || I.ctl[CTL_MODE] != MODE_INNER) return;
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
@@ -460,10 +473,14 @@ __global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ in
     I.rec[idx] = cross;
 }
 
+__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const TileList tl) {
+    const int n = tl.cnt ? *tl.cnt : tl.n;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_inner_tile(insts, tl.items[b]);
+}
+
 // after detect: rc = cross(low), rcF = diamond(low), scheme lowered
 // (src/detect.f90:60-72,219-234)
-__global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const Item* __restrict__ items) {
-    const Item it = items[blockIdx.x];
+__device__ __forceinline__ void lower_detect_tile(const Inst* __restrict__ insts, const Item it) {
     const Inst& I = insts[it.inst];
     if (!pass_ready(I)) return;
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
@@ -475,6 +492,11 @@ __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ i
         const int sc = I.scheme[idx];
         I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
     }
+}
+
+__global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const TileList tl) {
+    const int n = tl.cnt ? *tl.cnt : tl.n;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_detect_tile(insts, tl.items[b]);
 }
 
 // periodic MOOD sync: x sweep over every row, then y sweep over every column
@@ -633,9 +655,7 @@ __device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const
     return !(smooth1 || smooth2);
 }
 
-__global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Item* __restrict__ items,
-                                                int stage) {
-    const Item it = items[blockIdx.x];
+__device__ __forceinline__ void update_tile(const Inst* __restrict__ insts, const Item it, int stage) {
     const Inst& I = insts[it.inst];
     if (!pass_ready(I)) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
@@ -703,13 +723,18 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     }
 }
 
+__global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const TileList tl, int stage) {
+    const int n = tl.cnt ? *tl.cnt : tl.n;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) update_tile(insts, tl.items[b], stage);
+}
+
 // MOOD-loop control between passes, one thread per instance.
 // OP_BEGIN starts a stage; OP_NEXT decides, from the flags of the pass just
 // run, whether the instance continues its inner admissibility loop
 // (MODE_INNER), starts a new MOOD pass (MODE_NEWPASS) or leaves the loop.
-__global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int stage, int32_t* __restrict__ act_count) {
+__global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int stage, int32_t* __restrict__ scan_count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *act_count = 0;
+    if (k == 0) *scan_count = 0;
     if (k >= ninst) return;
     int32_t* c = insts[k].ctl;
     if (!insts[k].active) {
@@ -952,25 +977,27 @@ void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st) 
 }
 
 void launch_stage_init(const LaunchSet& L, hipStream_t st) {
+    if (L.npad <= 0) return;
     hipLaunchKernelGGL(k_stage_init, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
 }
 
-void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, hipStream_t st) {
-    hipLaunchKernelGGL(k_ctl, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst, op, stage, L.act_count);
+void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, int w, hipStream_t st) {
+    hipLaunchKernelGGL(k_ctl, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst, op, stage, L.act_count + w);
 }
 
-void launch_tile_scan(const LaunchSet& L, hipStream_t st) {
-    hipLaunchKernelGGL(k_tile_scan, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, L.act, L.act_count);
+void launch_tile_scan(const LaunchSet& L, int w, hipStream_t st) {
+    if (L.npad <= 0) return;
+    hipLaunchKernelGGL(k_tile_scan, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, L.act[w], L.act_count + w);
 }
 
-void launch_face(const LaunchSet& L, const Item* items, int n, int R, bool full, int stage, hipStream_t st) {
-    if (n <= 0) return;
-    const dim3 g = g1(n);
+void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
+    if (grid <= 0) return;
+    const dim3 g = g1(grid);
 #define HMHD_FACE(RR)                                                                                        \
     if (full)                                                                                                \
-        hipLaunchKernelGGL((k_face<RR, true>), g, dim3(face_threads(RR, true)), 0, st, L.insts, items, stage); \
+        hipLaunchKernelGGL((k_face<RR, true>), g, dim3(face_threads(RR, true)), 0, st, L.insts, tl, stage);    \
     else                                                                                                     \
-        hipLaunchKernelGGL((k_face<RR, false>), g, dim3(face_threads(RR, false)), 0, st, L.insts, items, stage);
+        hipLaunchKernelGGL((k_face<RR, false>), g, dim3(face_threads(RR, false)), 0, st, L.insts, tl, stage);
     if (R == 1) {
         HMHD_FACE(1)
     } else if (R == 2) {
@@ -981,19 +1008,19 @@ void launch_face(const LaunchSet& L, const Item* items, int n, int R, bool full,
 #undef HMHD_FACE
 }
 
-void launch_lower_inner(const LaunchSet& L, const Item* items, int n, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_lower_inner, g1(n), dim3(256), 0, st, L.insts, items);
+void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, hipStream_t st) {
+    if (grid <= 0) return;
+    hipLaunchKernelGGL(k_lower_inner, g1(grid), dim3(256), 0, st, L.insts, tl);
 }
 
-void launch_update(const LaunchSet& L, const Item* items, int n, int stage, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_update, g1(n), dim3(256), 0, st, L.insts, items, stage);
+void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st) {
+    if (grid <= 0) return;
+    hipLaunchKernelGGL(k_update, g1(grid), dim3(256), 0, st, L.insts, tl, stage);
 }
 
-void launch_lower_detect(const LaunchSet& L, const Item* items, int n, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_lower_detect, g1(n), dim3(256), 0, st, L.insts, items);
+void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st) {
+    if (grid <= 0) return;
+    hipLaunchKernelGGL(k_lower_detect, g1(grid), dim3(256), 0, st, L.insts, tl);
 }
 
 void launch_sync(const LaunchSet& L, hipStream_t st) {

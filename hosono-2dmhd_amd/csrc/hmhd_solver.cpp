@@ -72,7 +72,10 @@ int Engine::hip_check(hipError_t e, const char* what) {
 
 int Engine::ensure_stream() {
     if (st_) return HMHD_OK;
-    return hip_check(hipStreamCreate(&st_), "hipStreamCreate");
+    int rc = hip_check(hipStreamCreate(&st_), "hipStreamCreate");
+    for (int w = 0; w < 2 && !rc; w++)
+        rc = hip_check(hipEventCreateWithFlags(&ev_snap_[w], hipEventDisableTiming), "hipEventCreate");
+    return rc;
 }
 
 Engine::~Engine() {
@@ -87,7 +90,9 @@ Engine::~Engine() {
     void* host[] = {h_ctl_, h_cfl_, h_dt_, h_scheme_};
     for (void* p : host)
         if (p) (void)hipHostFree(p);
-    for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    for (auto e : ev_all_) (void)hipEventDestroy(e);
+    for (auto e : ev_snap_)
+        if (e) (void)hipEventDestroy(e);
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -290,7 +295,7 @@ int Engine::commit() {
     const size_t nctl = (size_t)CTL_N * n + 4;
     if ((rc = grow(&d_insts_, cap_insts_, n, false))) return rc;
     if ((rc = grow(&d_ctl_, cap_ctl_, nctl, false))) return rc;
-    if ((rc = grow(&h_ctl_, cap_hctl_, nctl, true))) return rc;
+    if ((rc = grow(&h_ctl_, cap_hctl_, 2 * nctl, true))) return rc;
     if ((rc = grow(&d_cfl_, cap_cfl_, 2 * n, false))) return rc;
     if ((rc = grow(&h_cfl_, cap_hcfl_, 2 * n, true))) return rc;
     if ((rc = grow(&d_dt_, cap_dt_, n, false))) return rc;
@@ -347,7 +352,7 @@ int Engine::commit() {
     if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
     if ((rc = put(band, &d_band_, cap_band_))) return rc;
     if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
-    if ((rc = grow(&d_act_, cap_act_, pad.size() + 1, false))) return rc;
+    if ((rc = grow(&d_act_, cap_act_, 2 * (pad.size() + 1), false))) return rc;
     if ((rc = grow(&d_part_, cap_part_, 2 * cell.size() + 2, false))) return rc;
     if ((rc = grow(&d_stage_, cap_stage_, 10 * max_pad, false))) return rc;
     if ((rc = grow(&d_stage_i_, cap_stage_i_, max_pad, false))) return rc;
@@ -359,8 +364,10 @@ int Engine::commit() {
     L.nband = any_periodic ? (int)band.size() : 0;
     L.cell = d_cell_;
     L.ncell = (int)cell.size();
-    L.act = d_act_;
+    L.act[0] = d_act_;
+    L.act[1] = d_act_ + pad.size() + 1;
     L.act_count = d_ctl_ + (size_t)CTL_N * n;
+    est_act_ = L.npad;
     L.cflpart = d_part_;
     ctl_.assign((size_t)CTL_N * n, 0);
     dirty_ = false;
@@ -387,52 +394,74 @@ int Engine::set_gp(int64_t id, const double* pv, const double* qw) {
 // ------------------------------------------------------------------ //
 // profiling: events bracket each launch of a kernel family
 
+hipEvent_t Engine::ev_get() {
+    if (ev_free_.empty()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ev_all_.push_back(e);
+        return e;
+    }
+    hipEvent_t e = ev_free_.back();
+    ev_free_.pop_back();
+    return e;
+}
+
 void Engine::prof_begin(int fam) {
     if (!(prof_mask >> fam & 1u)) return;
-    if (ev_next_ + 2 > ev_pool_.size()) {
-        for (int k = 0; k < 64; k++) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return;
-            ev_pool_.push_back(e);
-        }
-    }
-    ev_a_ = ev_pool_[ev_next_++];
+    ev_a_ = ev_get();
     ev_fam_ = fam;
-    (void)hipEventRecord(ev_a_, st_);
+    if (ev_a_) (void)hipEventRecord(ev_a_, st_);
 }
 
 void Engine::prof_end(int fam) {
     if (!(prof_mask >> fam & 1u) || ev_fam_ != fam || !ev_a_) return;
-    hipEvent_t b = ev_pool_[ev_next_++];
+    hipEvent_t b = ev_get();
+    if (!b) return;
     (void)hipEventRecord(b, st_);
-    ev_pending_.push_back({fam, {ev_a_, b}});
+    ev_pending_.push_back(EvPair{fam, ev_a_, b});
     ev_a_ = nullptr;
     ev_fam_ = -1;
 }
 
-void Engine::prof_collect() {
-    for (auto& p : ev_pending_) {
+// add up the timed launches that have finished (all of them when the
+// stream is known to be idle)
+void Engine::prof_collect(bool wait_all) {
+    while (!ev_pending_.empty()) {
+        EvPair& p = ev_pending_.front();
+        if (!wait_all && hipEventQuery(p.b) != hipSuccess) break;
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
-            prof_ms[p.first] += ms;
-            prof_n[p.first] += 1;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            prof_ms[p.fam] += ms;
+            prof_n[p.fam] += 1;
         }
+        ev_free_.push_back(p.a);
+        ev_free_.push_back(p.b);
+        ev_pending_.pop_front();
     }
-    ev_pending_.clear();
-    ev_next_ = 0;
 }
 
 // ------------------------------------------------------------------ //
 
-int Engine::read_ctl() {
-    const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 1;
-    int rc = hip_check(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(int32_t) * nctl, hipMemcpyDeviceToHost, st_),
+// copy the control words and both active-list counts to host slot w once
+// the work enqueued so far has run
+int Engine::snapshot(int w) {
+    const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 2;
+    int rc = hip_check(hipMemcpyAsync(h_ctl_ + w * nctl, d_ctl_, sizeof(int32_t) * nctl, hipMemcpyDeviceToHost, st_),
                        "hipMemcpyAsync(ctl)");
     if (rc) return rc;
-    if ((rc = hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize"))) return rc;
-    std::memcpy(ctl_.data(), h_ctl_, sizeof(int32_t) * CTL_N * n);
-    act_n_ = h_ctl_[CTL_N * n];
-    prof_collect();
+    return hip_check(hipEventRecord(ev_snap_[w], st_), "hipEventRecord");
+}
+
+// wait for snapshot w; its control words become ctl_, and the count of the
+// list that snapshot's iteration wrote becomes act_n_
+int Engine::read_snapshot(int w) {
+    const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 2;
+    int rc = hip_check(hipEventSynchronize(ev_snap_[w]), "hipEventSynchronize");
+    if (rc) return rc;
+    const int32_t* h = h_ctl_ + w * nctl;
+    std::memcpy(ctl_.data(), h, sizeof(int32_t) * CTL_N * n);
+    act_n_ = h[CTL_N * n + w];
+    prof_collect(false);
     if (prof_mask >> PF_TILES & 1u) {
         prof_n[PF_TILES] += act_n_;  // active tiles summed over MOOD iterations
         prof_ms[PF_TILES] += 1.0;    // MOOD iterations (host round trips)
@@ -441,35 +470,70 @@ int Engine::read_ctl() {
 }
 
 // templated kernels run once per GP radius present (tiles of other radii return)
-void Engine::launch_face_groups(const Item* items, int n, bool full, int stage) {
+void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
     for (int R = 1; R <= kMaxR; R++)
-        if (has_R_[R]) launch_face(L, items, n, R, full, stage, st_);
+        if (has_R_[R]) launch_face(L, tl, grid, R, full, stage, st_);
 }
 
-// one RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active solve
-int Engine::run_stage(int s) {
-    const int n = (int)inst_.size();
-    // first pass: every tile, every face at the top radius
-    launch_ctl(L, n, OP_BEGIN, s, st_);
-    launch_stage_init(L, st_);
-    prof_begin(PF_FACE_FULL);
-    launch_face_groups(L.pad, L.npad, true, s);
-    prof_end(PF_FACE_FULL);
+// MOOD iteration it >= 1 of stage s: the active tiles listed by iteration
+// it - 1 (list (it - 1) & 1), its own scan writing list it & 1, then a
+// snapshot into host slot it & 1.  `grid` blocks stride over the list.
+int Engine::enqueue_iteration(int s, int it, int grid) {
+    const int n = (int)inst_.size(), r = (it - 1) & 1, w = it & 1;
+    const TileList act{L.act[r], 0, L.act_count + r};
+    launch_ctl(L, n, OP_NEXT, s, w, st_);
+    prof_begin(PF_RECON);
+    launch_lower_inner(L, act, grid, st_);
+    prof_end(PF_RECON);
+    prof_begin(PF_FACE_FLUX);
+    launch_face_groups(act, grid, false, s);
+    prof_end(PF_FACE_FLUX);
     prof_begin(PF_UPDATE);
-    launch_update(L, L.pad, L.npad, s, st_);
+    launch_update(L, act, grid, s, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
-    launch_lower_detect(L, L.pad, L.npad, st_);
+    launch_lower_detect(L, act, grid, st_);
     launch_sync(L, st_);
-    launch_tile_scan(L, st_);
+    launch_tile_scan(L, w, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    if ((rc = read_ctl())) return rc;
-    for (int iter = 0;; iter++) {
+    return snapshot(w);
+}
+
+// One RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active
+// solve.  The loop runs one iteration ahead of the host: iteration it + 1
+// is enqueued before the host has read whether iteration it left work.  A
+// speculative iteration after the last needed one is a no-op (k_ctl ends
+// every solve's loop when no flag is set, and every kernel then returns),
+// so the results and the pass counts are those of the plain loop.
+int Engine::run_stage(int s) {
+    const int n = (int)inst_.size();
+    const TileList all{L.pad, L.npad, nullptr};
+    // iteration 0, the first pass: every tile, every face at the top radius
+    launch_ctl(L, n, OP_BEGIN, s, 0, st_);
+    launch_stage_init(L, st_);
+    prof_begin(PF_FACE_FULL);
+    launch_face_groups(all, L.npad, true, s);
+    prof_end(PF_FACE_FULL);
+    prof_begin(PF_UPDATE);
+    launch_update(L, all, L.npad, s, st_);
+    prof_end(PF_UPDATE);
+    prof_begin(PF_FLAGS);
+    launch_lower_detect(L, all, L.npad, st_);
+    launch_sync(L, st_);
+    launch_tile_scan(L, 0, st_);
+    prof_end(PF_FLAGS);
+    int rc = hip_check(hipGetLastError(), "kernel launch");
+    if (rc) return rc;
+    if ((rc = snapshot(0))) return rc;
+    auto grid_for = [&](int listed) { return std::min(L.npad, listed + listed / 2 + 16); };
+    if ((rc = enqueue_iteration(s, 1, grid_for(est_act_)))) return rc;
+    for (int it = 0;; it++) {
+        if ((rc = read_snapshot(it & 1))) return rc;
         // every continuing iteration lowers at least one scheme value, so
         // the loop ends; the cap only turns a defect into an error
-        if (iter > 64 * (kMaxR + 1) * 64) {
+        if (it > 64 * (kMaxR + 1) * 64) {
             err = "MOOD loop did not converge";
             return HMHD_ERR_DEVICE;
         }
@@ -478,27 +542,11 @@ int Engine::run_stage(int s) {
             const int32_t* c = &ctl_[(size_t)CTL_N * k];
             if (c[CTL_RUN] && (c[CTL_ANY_INADM] || c[CTL_ANY_LOWER])) more = true;
         }
-        if (!more) break;
-        // later passes and inner iterations: only the active tiles the
-        // previous iteration's scan listed
-        const int na = act_n_;
-        launch_ctl(L, n, OP_NEXT, s, st_);
-        prof_begin(PF_RECON);
-        launch_lower_inner(L, L.act, na, st_);
-        prof_end(PF_RECON);
-        prof_begin(PF_FACE_FLUX);
-        launch_face_groups(L.act, na, false, s);
-        prof_end(PF_FACE_FLUX);
-        prof_begin(PF_UPDATE);
-        launch_update(L, L.act, na, s, st_);
-        prof_end(PF_UPDATE);
-        prof_begin(PF_FLAGS);
-        launch_lower_detect(L, L.act, na, st_);
-        launch_sync(L, st_);
-        launch_tile_scan(L, st_);
-        prof_end(PF_FLAGS);
-        if ((rc = hip_check(hipGetLastError(), "kernel launch"))) return rc;
-        if ((rc = read_ctl())) return rc;
+        if (it == 0) est_act_ = act_n_;
+        if (!more) break;  // iteration it + 1, already enqueued, does nothing
+        // iteration it + 2 reads the list iteration it + 1 writes; size its
+        // grid from the latest count the host has seen
+        if ((rc = enqueue_iteration(s, it + 2, grid_for(act_n_)))) return rc;
     }
     if (record_schemes) {
         size_t off = 0;
@@ -596,7 +644,7 @@ int Engine::cfl(std::vector<double>& dt_out) {
                         "hipMemcpyAsync(cfl)")))
         return rc;
     if ((rc = sync())) return rc;
-    prof_collect();
+    prof_collect(true);
     dt_out.clear();
     for (int k = 0; k < n; k++) {
         if (!inst_[k].active) continue;
@@ -656,7 +704,9 @@ int Engine::download(int64_t id, double* U, double* V, int32_t* scheme) {
 
 int Engine::sync() {
     if (!st_) return HMHD_OK;
-    return hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+    const int rc = hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+    if (!rc) prof_collect(true);
+    return rc;
 }
 
 }  // namespace hmhd

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <deque>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -92,12 +93,14 @@ class Engine {
     int hip_check(hipError_t e, const char* what);
     int ensure_stream();
     int commit();
-    int read_ctl();
+    int snapshot(int w);
+    int read_snapshot(int w);
+    int enqueue_iteration(int s, int it, int grid);
     int run_stage(int s);
     void prof_begin(int fam);
     void prof_end(int fam);
-    void prof_collect();
-    void launch_face_groups(const Item* items, int n, bool full, int in);
+    void prof_collect(bool wait_all);
+    void launch_face_groups(TileList tl, int grid, bool full, int stage);
     template <class T>
     int grow(T** p, size_t& cap, size_t need, bool host);
 
@@ -113,8 +116,9 @@ class Engine {
     size_t cap_insts_ = 0;
     int32_t* d_ctl_ = nullptr;  // CTL_N words per solve, then the active-tile count
     size_t cap_ctl_ = 0;
-    int32_t* h_ctl_ = nullptr;  // pinned mirror
+    int32_t* h_ctl_ = nullptr;  // pinned, two snapshot slots
     size_t cap_hctl_ = 0;
+    hipEvent_t ev_snap_[2] = {nullptr, nullptr};
     unsigned long long* d_cfl_ = nullptr;  // two maxima per solve
     size_t cap_cfl_ = 0;
     unsigned long long* h_cfl_ = nullptr;
@@ -129,7 +133,7 @@ class Engine {
     size_t cap_band_ = 0;
     Item* d_cell_ = nullptr;
     size_t cap_cell_ = 0;
-    Item* d_act_ = nullptr;
+    Item* d_act_ = nullptr;  // the two active lists
     size_t cap_act_ = 0;
     unsigned long long* d_part_ = nullptr;
     size_t cap_part_ = 0;
@@ -140,15 +144,20 @@ class Engine {
     int32_t* d_stage_i_ = nullptr;
     size_t cap_stage_i_ = 0;
     LaunchSet L{};
-    int act_n_ = 0;
+    int act_n_ = 0;    // length of the active list in the latest snapshot
+    int est_act_ = 0;  // active tiles after the previous stage's first pass
     bool has_R_[kMaxR + 1] = {false};
     std::vector<int32_t> ctl_;
 
-    std::vector<hipEvent_t> ev_pool_;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending_;
-    size_t ev_next_ = 0;
+    struct EvPair {
+        int fam;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> ev_free_, ev_all_;
+    std::deque<EvPair> ev_pending_;  // in stream order
     hipEvent_t ev_a_ = nullptr;
     int ev_fam_ = -1;
+    hipEvent_t ev_get();
 };
 
 }  // namespace hmhd
