@@ -67,6 +67,7 @@ struct Inst {
     uint8_t* low;
     uint8_t* rec;
     uint8_t* faceRR;  // [4][my][mx]
+    uint8_t* fluxRR;  // [2][my][mx] radius of the last flux of the x / y face
     int32_t* ctl;
     unsigned long long* cflmax;  // [2] bit patterns of the max speeds
 };

@@ -18,10 +18,11 @@
 // radius, admissibility checked on the way.  Every later iteration (a new
 // MOOD pass, or another round of the inner admissibility loop when a
 // reconstructed state was inadmissible) runs k_lower_inner and k_face on the
-// active tiles only: the flagged cells are reconstructed and checked, and
-// the faces the reference's getFluxes_ would recompute are recomputed, each
-// side from the radius it was last reconstructed with.  k_tile_scan lists
-// the tiles the next iteration needs.
+// active tiles only: of the faces the reference would rebuild (flagged
+// sides, fluxes getFluxes_ recomputes), those whose inputs changed are
+// recomputed, each side from the radius it was last reconstructed with,
+// and the others only repeat the recomputation's side effects.
+// k_tile_scan lists the tiles the next iteration needs.
 #include <hip/hip_runtime.h>
 
 #include "hmhd_internal.hpp"
@@ -220,20 +221,20 @@ __device__ __forceinline__ void face_iter(const Phys& ph, int set, const A& aL, 
 }
 
 // ------------------------------------------------------------------ //
-// face kernel, one block per 16 x 16 tile.  Padded cell c owns the x-face
-// at i-1/2 (the east state of its west neighbour and its own west state)
-// and the y-face at j-1/2 (north state of the south neighbour, own south
-// state).  In the later passes at R <= 2 the block has 2 x 256 threads and
-// thread (d, c) does face d of cell c: splitting the directions halves a
-// block's latency, which bounds those sparse passes.  The first pass keeps
-// 256 threads doing both faces (two blocks per CU overlap one's LDS fill
-// with the other's arithmetic), as does R = 3, whose registers for one
-// direction already fill a thread at two waves per SIMD.
-// Cells of [s-2, e+3]^2 take part (tiles of the padded grid, the same
-// tiles every per-cell kernel uses).
-// FULL: the first pass of a stage -- every state of [s-2, e+2]^2 at radius
-// R, checked, and every flux of [s-1, e+2]^2.  Otherwise: one iteration of
-// the inner admissibility loop fused with the flux recomputation.
+// face kernel, one block per 16 x 16 tile (blocks stride over the tile
+// list).  Padded cell c owns the x-face at i-1/2 (the east state of its
+// west neighbour and its own west state) and the y-face at j-1/2 (north
+// state of the south neighbour, own south state); cells of [s-2, e+3]^2
+// take part (tiles of the padded grid, the same tiles every per-cell kernel
+// uses).
+// FULL, the first pass of a stage: every state of [s-2, e+2]^2 at radius
+// R, checked, and every flux of [s-1, e+2]^2; 256 threads, one cell's two
+// faces each (two blocks per CU overlap one's LDS fill with the other's
+// arithmetic).
+// Later passes (face_tile_iter): one iteration of the inner admissibility
+// loop fused with the flux recomputation, over a compacted list of the
+// faces whose inputs changed; at R <= 2 the block has 512 threads so a
+// sparse pass costs one round of lanes.
 // Inadmissibility is recorded per face side in four byte planes, so each
 // byte has a single writer.  Register budget: two waves per SIMD at R <= 2;
 // R = 3 keeps one (its stencils would spill at two).
@@ -246,19 +247,16 @@ constexpr int face_lds_cells() {
     return (kFaceTX + 2 * R + 1) * (kFaceTY + 2 * R + 1);
 }
 
-template <int R, bool FULL>
-__device__ __forceinline__ void face_tile(const Inst* __restrict__ insts, const Item it, int stage,
+template <int R>
+__device__ __forceinline__ void face_tile_full(const Inst* __restrict__ insts, const Item it, int stage,
                                           double* __restrict__ sU) {
-    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, FULL);
+    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, true);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
     const Inst& I = insts[it.inst];
     if (I.R != R) return;
-    if (!FULL && !I.ctl[CTL_RUN]) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    // 0: x-faces, 1: y-faces, 2: both (wave-uniform)
-    const int dir = NT > TX * TY ? (int)(threadIdx.x / (TX * TY)) : 2;
-    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
     const int i0 = it.tx * TX, j0 = it.ty * TY;
     const int i = i0 + lx, j = j0 + ly;
     const bool in_dom = i >= s - 2 && i <= ex + 3 && j >= s - 2 && j <= ey + 3;
@@ -269,22 +267,8 @@ __device__ __forceinline__ void face_tile(const Inst* __restrict__ insts, const 
     const bool cellW = in_dom && i >= s - 1 && j <= ey + 2;
     const bool cellS = in_dom && j >= s - 1 && i <= ex + 2;
 
-    bool recC = false, recW = false, recS = false, fx = false, fy = false;
-    if (FULL) {
-        recC = cellC;
-        recW = cellW;
-        recS = cellS;
-        fx = fy = flux_cell;
-    } else {
-        const uint8_t* flag = I.ctl[CTL_MODE] == MODE_NEWPASS ? I.rc : I.rec;
-        recC = cellC && flag[idx];
-        recW = cellW && flag[idx - 1];
-        recS = cellS && flag[idx - mx];
-        fx = flux_cell && (I.rc[idx] || I.rc[idx - 1]);
-        fy = flux_cell && (I.rc[idx] || I.rc[idx - mx]);
-    }
-    const bool do_x = dir != 1 && (recC || recW || fx), do_y = dir != 0 && (recC || recS || fy);
-    if (!FULL && !__syncthreads_or(do_x || do_y)) return;
+    const bool recC = cellC, recW = cellW, recS = cellS, fx = flux_cell, fy = flux_cell;
+    const bool do_x = recC || recW || fx, do_y = recC || recS || fy;
 
     // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
     {
@@ -309,7 +293,7 @@ __device__ __forceinline__ void face_tile(const Inst* __restrict__ insts, const 
     uint8_t* const bad = I.inadm;  // planes [face][my][mx]
     bool any_bad = false;
 
-    if (FULL) {
+    {
         constexpr int NQ = gp_nq(R);
         // x-face: east state of (i-1, j) | west state of (i, j)
         if (do_x) {
@@ -332,6 +316,7 @@ __device__ __forceinline__ void face_tile(const Inst* __restrict__ insts, const 
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
                 I.FLB[idx] = acc[MAGY];
+                I.fluxRR[idx] = (uint8_t)R;
             }
         }
         // y-face: north state of (i, j-1) | south state of (i, j)
@@ -355,54 +340,138 @@ __device__ __forceinline__ void face_tile(const Inst* __restrict__ insts, const 
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
                 I.FLB[fplane + idx] = acc[MAGX];
+                I.fluxRR[fplane + idx] = (uint8_t)R;
             }
         }
-    } else {
-        const int sc = I.scheme[idx];
-        if (do_x) {
-            const int sw = I.scheme[idx - 1];
-            const int rr_now = sc < sw ? sc : sw;
-            bool badL, badR;
-            double acc[NCONS];
-            face_iter<R, 0>(ph, set, aW, aC, FACE_E, FACE_W, recW, recC, rr_now, I.faceRR[FACE_E * fplane + idx - 1],
-                            I.faceRR[FACE_W * fplane + idx], fx, acc, badL, badR);
-            if (recW) {
-                I.faceRR[FACE_E * fplane + idx - 1] = (uint8_t)rr_now;
-                bad[FACE_E * fplane + idx - 1] = badL;
+    }
+    if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
+}
+
+// A later MOOD iteration on one tile.  Every face the reference's pass
+// would touch -- a side to re-reconstruct and check, a flux to recompute
+// -- is classified first.  A face whose sides would be rebuilt at the radii
+// they already have, and whose flux would be recomputed at the radius it
+// was last computed with, would reproduce the stored states, admissibility
+// flags and flux bit for bit (the stage input U and the GP tables do not
+// change within a stage), so only the recomputation's side effects are
+// repeated: the raw-flux copy into FLB and the inadmissibility report.  The
+// remaining faces are compacted, x-faces then y-faces from a wave boundary,
+// so lanes do not idle behind untouched faces.
+template <int R>
+__device__ __forceinline__ void face_tile_iter(const Inst* __restrict__ insts, const Item it, int stage,
+                                               double* __restrict__ sU, uint16_t* __restrict__ lst,
+                                               int* __restrict__ lcnt) {
+    constexpr int TX = kFaceTX, TY = kFaceTY, NC = TX * TY, H = R + 1, NT = face_threads(R, false);
+    constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
+
+    const Inst& I = insts[it.inst];
+    if (I.R != R || !I.ctl[CTL_RUN]) return;
+    const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
+    const int i0 = it.tx * TX, j0 = it.ty * TY;
+    const size_t fplane = (size_t)mx * I.my;
+    const uint8_t* flag = I.ctl[CTL_MODE] == MODE_NEWPASS ? I.rc : I.rec;
+    if (threadIdx.x < 2) lcnt[threadIdx.x] = 0;
+    __syncthreads();
+
+    bool any_bad = false;
+    for (int f = threadIdx.x; f < 2 * NC; f += NT) {
+        const int d = f / NC, c = f - d * NC;
+        const int i = i0 + c % TX, j = j0 + c / TX;
+        if (i < s - 2 || i > ex + 3 || j < s - 2 || j > ey + 3) continue;
+        const size_t idx = (size_t)j * mx + i, nb = d == 0 ? idx - 1 : idx - mx;
+        // the face's cells within the reconstruction range [s-2, e+2]^2
+        const bool cellR = i <= ex + 2 && j <= ey + 2;
+        const bool cellL = d == 0 ? (i >= s - 1 && j <= ey + 2) : (j >= s - 1 && i <= ex + 2);
+        const bool recR = cellR && flag[idx], recL = cellL && flag[nb];
+        const bool fl = i >= s - 1 && i <= ex + 2 && j >= s - 1 && j <= ey + 2 && (I.rc[idx] || I.rc[nb]);
+        if (!(recL || recR || fl)) continue;
+        const int fL = d == 0 ? FACE_E : FACE_N, fR = d == 0 ? FACE_W : FACE_S;
+        const int sc = I.scheme[idx], sn = I.scheme[nb];
+        const int rr_now = sc < sn ? sc : sn;
+        const bool keepL = !recL || I.faceRR[fL * fplane + nb] == rr_now;
+        const bool keepR = !recR || I.faceRR[fR * fplane + idx] == rr_now;
+        const bool keepF = !fl || I.fluxRR[d * fplane + idx] == rr_now;
+        if (keepL && keepR && keepF) {
+            if (fl) I.FLB[d * fplane + idx] = I.FR[(d * NCONS + (d == 0 ? MAGY : MAGX)) * fplane + idx];
+            any_bad |= (recL && I.inadm[fL * fplane + nb]) || (recR && I.inadm[fR * fplane + idx]);
+        } else {
+            const int slot = atomicAdd(&lcnt[d], 1);
+            lst[d * NC + slot] = (uint16_t)(c | (recL ? 0x100 : 0) | (recR ? 0x200 : 0) | (fl ? 0x400 : 0));
+        }
+    }
+    __syncthreads();
+    const int kx = lcnt[0], ky = lcnt[1], xw = (kx + 63) & ~63;
+    if (kx + ky > 0) {
+        // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
+        const double* __restrict__ U = I.U[stage_bufs(I.rot, stage).in];
+        const size_t plane = (size_t)I.nx * I.ny;
+        const int oi = i0 - H, oj = j0 - H;
+        for (int c = threadIdx.x; c < LP; c += NT) {
+            const int cy = c / LX, cx = c - cy * LX;
+            const int kxx = map_axis(oi + cx, s, I.nx, I.bc), kyy = map_axis(oj + cy, s, I.ny, I.bc);
+            const double* src = U + (size_t)kyy * I.nx + kxx;
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) sU[v * LP + c] = src[v * plane];
+        }
+    }
+    __syncthreads();
+    const Phys ph = phys_of(I);
+    const int set = I.gpset;
+    // x-faces occupy slots [0, kx), y-faces [xw, xw + ky): each wave does
+    // faces of one direction
+    for (int slot = threadIdx.x; slot < xw + ky; slot += NT) {
+        const bool isx = slot < kx;
+        if (!isx && slot < xw) continue;
+        const int e = isx ? lst[slot] : lst[NC + slot - xw];
+        const int c = e & 0xff, lx = c % TX, ly = c / TX;
+        const bool recL = e & 0x100, recR = e & 0x200, fl = e & 0x400;
+        const size_t idx = (size_t)(j0 + ly) * mx + (i0 + lx);
+        const int cc = (ly + H) * LX + (lx + H);
+        const LdsAcc<LX, LP> aR{sU + cc};
+        bool badL, badR;
+        double acc[NCONS];
+        if (isx) {
+            const size_t nb = idx - 1;
+            const int sc = I.scheme[idx], sn = I.scheme[nb];
+            const int rr_now = sc < sn ? sc : sn;
+            face_iter<R, 0>(ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, FACE_E, FACE_W, recL, recR, rr_now,
+                            I.faceRR[FACE_E * fplane + nb], I.faceRR[FACE_W * fplane + idx], fl, acc, badL, badR);
+            if (recL) {
+                I.faceRR[FACE_E * fplane + nb] = (uint8_t)rr_now;
+                I.inadm[FACE_E * fplane + nb] = badL;
             }
-            if (recC) {
+            if (recR) {
                 I.faceRR[FACE_W * fplane + idx] = (uint8_t)rr_now;
-                bad[FACE_W * fplane + idx] = badR;
+                I.inadm[FACE_W * fplane + idx] = badR;
             }
-            any_bad |= (recW && badL) || (recC && badR);
-            if (fx) {
+            if (fl) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
                 I.FLB[idx] = acc[MAGY];
+                I.fluxRR[idx] = (uint8_t)rr_now;
             }
-        }
-        if (do_y) {
-            const int ss = I.scheme[idx - mx];
-            const int rr_now = sc < ss ? sc : ss;
-            bool badL, badR;
-            double acc[NCONS];
-            face_iter<R, 1>(ph, set, aS, aC, FACE_N, FACE_S, recS, recC, rr_now, I.faceRR[FACE_N * fplane + idx - mx],
-                            I.faceRR[FACE_S * fplane + idx], fy, acc, badL, badR);
-            if (recS) {
-                I.faceRR[FACE_N * fplane + idx - mx] = (uint8_t)rr_now;
-                bad[FACE_N * fplane + idx - mx] = badL;
+        } else {
+            const size_t nb = idx - mx;
+            const int sc = I.scheme[idx], sn = I.scheme[nb];
+            const int rr_now = sc < sn ? sc : sn;
+            face_iter<R, 1>(ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, FACE_N, FACE_S, recL, recR, rr_now,
+                            I.faceRR[FACE_N * fplane + nb], I.faceRR[FACE_S * fplane + idx], fl, acc, badL, badR);
+            if (recL) {
+                I.faceRR[FACE_N * fplane + nb] = (uint8_t)rr_now;
+                I.inadm[FACE_N * fplane + nb] = badL;
             }
-            if (recC) {
+            if (recR) {
                 I.faceRR[FACE_S * fplane + idx] = (uint8_t)rr_now;
-                bad[FACE_S * fplane + idx] = badR;
+                I.inadm[FACE_S * fplane + idx] = badR;
             }
-            any_bad |= (recS && badL) || (recC && badR);
-            if (fy) {
+            if (fl) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
                 I.FLB[fplane + idx] = acc[MAGX];
+                I.fluxRR[fplane + idx] = (uint8_t)rr_now;
             }
         }
+        any_bad |= (recL && badL) || (recR && badR);
     }
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
@@ -411,10 +480,15 @@ template <int R, bool FULL>
 __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
     const Inst* __restrict__ insts, const TileList tl, int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
+    __shared__ uint16_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
+    __shared__ int lcnt[2];
     const int n = tl.cnt ? *tl.cnt : tl.n;
     for (int b = blockIdx.x; b < n; b += gridDim.x) {
         if (b != (int)blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
-        face_tile<R, FULL>(insts, tl.items[b], stage, sU);
+        if (FULL)
+            face_tile_full<R>(insts, tl.items[b], stage, sU);
+        else
+            face_tile_iter<R>(insts, tl.items[b], stage, sU, lst, lcnt);
     }
 }
 

@@ -131,7 +131,7 @@ int Engine::check_config(const hmhd_config& c, std::string& err) {
 }
 
 // layout of one solve's device block: U[3], V0, FR, FLB, then the byte
-// planes scheme, rc, rcF, low, rec, faceRR[4], inadm[4]
+// planes scheme, rc, rcF, low, rec, faceRR[4], inadm[4], fluxRR[2]
 size_t Engine::footprint(const hmhd_config& c) {
     const int ngc = c.R + 2;
     const size_t nin = (size_t)c.nx * c.ny, npad = (size_t)(c.nx + 2 * ngc) * (c.ny + 2 * ngc);
@@ -139,7 +139,7 @@ size_t Engine::footprint(const hmhd_config& c) {
     b += 4 * align_up(sizeof(double) * NCONS * nin);
     b += align_up(sizeof(double) * 2 * NCONS * npad);
     b += align_up(sizeof(double) * 2 * npad);
-    b += 5 * align_up(npad) + 2 * align_up(4 * npad);
+    b += 5 * align_up(npad) + 2 * align_up(4 * npad) + align_up(2 * npad);
     return b;
 }
 
@@ -183,6 +183,7 @@ int64_t Engine::add(const hmhd_config& c) {
     d.rec = reinterpret_cast<uint8_t*>(take(npad));
     d.faceRR = reinterpret_cast<uint8_t*>(take(4 * npad));
     d.inadm = reinterpret_cast<uint8_t*>(take(4 * npad));
+    d.fluxRR = reinterpret_cast<uint8_t*>(take(2 * npad));
     d.nx = c.nx;
     d.ny = c.ny;
     d.ngc = h.ngc;
