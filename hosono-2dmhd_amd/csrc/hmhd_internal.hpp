@@ -52,7 +52,7 @@ struct Inst {
     int32_t cell_off, cell_cnt;  // CFL blocks of this instance
     int32_t rot;                 // U[rot] holds U^n; stage buffers rotate from it
     int32_t v0;                  // the next CFL reads the uploaded primitives V0
-    int32_t pad0, pad1;
+    int32_t tile_off, ntx;       // this instance's tiles in the candidate marks; tiles per row
     double dx, dy, mindxdy, maxdxdy;
     double gamma, small_p, small_rho, cfl;
     double dtdx, dtdy;
@@ -102,6 +102,7 @@ struct LaunchSet {
     int ncell;
     Item* act[2];        // active tiles, written by alternate MOOD iterations
     int32_t* act_count;  // [2] their counts
+    uint32_t* marks;     // per tile: stamp of the last iteration that may change its flags
     unsigned long long* cflpart;  // two partial maxima per CFL block
 };
 

@@ -530,9 +530,35 @@ __device__ __forceinline__ uint8_t gather_inadm(const Inst& I, int i, int j, int
 
 // lowering of every inadmissible cell, as a gather (src/reconstruct.f90:146-171
 // with src/gridBlock.f90:194-235): rc |= cross, rcF |= diamond, rec = cross
-__device__ __forceinline__ void lower_inner_tile(const Inst* __restrict__ insts, const Item it) {
+// Tiles whose flags this iteration may change, for the scan that ends it:
+// within one tile of an active tile (every kernel writes flags of its own
+// tile's cells, the face kernel also the east / north sides of the cells
+// west / south of it), and, for a periodic solve, around the images of the
+// tile by +-N in x and y, where the MOOD sync copies guard-band flags.
+// A tile outside this set had no flag in its footprint after the previous
+// iteration and gains none in this one, so its scan would come out empty.
+__device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, uint32_t* __restrict__ marks,
+                                                uint32_t stamp) {
+    const int t = threadIdx.x;
+    if (t >= 9 || (t > 0 && I.bc != BC_PERIODIC)) return;
+    const int sx = (t % 3 == 1) ? I.nx : (t % 3 == 2 ? -I.nx : 0);
+    const int sy = (t / 3 == 1) ? I.ny : (t / 3 == 2 ? -I.ny : 0);
+    const int nty = (I.my + kCellT - 1) / kCellT;
+    const int x0 = it.tx * kCellT + sx - kCellT, x1 = it.tx * kCellT + sx + 2 * kCellT - 1;
+    const int y0 = it.ty * kCellT + sy - kCellT, y1 = it.ty * kCellT + sy + 2 * kCellT - 1;
+    if (x1 < 0 || y1 < 0 || x0 >= I.mx || y0 >= I.my) return;
+    const int ta = x0 < 0 ? 0 : x0 / kCellT, tb = x1 >= I.mx ? I.ntx - 1 : x1 / kCellT;
+    const int ua = y0 < 0 ? 0 : y0 / kCellT, ub = y1 >= I.my ? nty - 1 : y1 / kCellT;
+    for (int u = ua; u <= ub; u++)
+        for (int v = ta; v <= tb; v++) marks[I.tile_off + u * I.ntx + v] = stamp;
+}
+
+__device__ __forceinline__ void lower_inner_tile(const Inst* __restrict__ insts, const Item it,
+                                                 uint32_t* __restrict__ marks, uint32_t stamp) {
     const Inst& I = insts[it.inst];
-    if (!I.ctl[CTL_RUN] || I.ctl[CTL_MODE] != MODE_INNER) return;
+    if (!I.ctl[CTL_RUN]) return;
+    mark_candidates(I, it, marks, stamp);
+    if (I.ctl[CTL_MODE] != MODE_INNER) return;
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i >= I.mx || j >= I.my) return;
     const size_t idx = (size_t)j * I.mx + i;
@@ -547,9 +573,10 @@ __device__ __forceinline__ void lower_inner_tile(const Inst* __restrict__ insts,
     I.rec[idx] = cross;
 }
 
-__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const TileList tl) {
+__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const TileList tl,
+                                                     uint32_t* __restrict__ marks, uint32_t stamp) {
     const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_inner_tile(insts, tl.items[b]);
+    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_inner_tile(insts, tl.items[b], marks, stamp);
 }
 
 // after detect: rc = cross(low), rcF = diamond(low), scheme lowered
@@ -842,9 +869,11 @@ __global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int sta
 // when its footprint grown by two cells holds such a cell.  Appended in any
 // order: each tile's work is independent of the others'.
 __global__ __launch_bounds__(256) void k_tile_scan(const Inst* __restrict__ insts, const Item* __restrict__ items,
-                                                   Item* __restrict__ act, int32_t* __restrict__ act_count) {
+                                                   Item* __restrict__ act, int32_t* __restrict__ act_count,
+                                                   const uint32_t* __restrict__ marks, uint32_t stamp) {
     const Item it = items[blockIdx.x];
     const Inst& I = insts[it.inst];
+    if (stamp != 0 && marks[I.tile_off + it.ty * I.ntx + it.tx] != stamp) return;
     const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
     constexpr int W = kCellT + 4;
     bool any = false;
@@ -1059,9 +1088,10 @@ void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, int w, hipStre
     hipLaunchKernelGGL(k_ctl, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst, op, stage, L.act_count + w);
 }
 
-void launch_tile_scan(const LaunchSet& L, int w, hipStream_t st) {
+void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st) {
     if (L.npad <= 0) return;
-    hipLaunchKernelGGL(k_tile_scan, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, L.act[w], L.act_count + w);
+    hipLaunchKernelGGL(k_tile_scan, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, L.act[w], L.act_count + w, L.marks,
+                       stamp);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
@@ -1082,9 +1112,9 @@ void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, in
 #undef HMHD_FACE
 }
 
-void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, hipStream_t st) {
+void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st) {
     if (grid <= 0) return;
-    hipLaunchKernelGGL(k_lower_inner, g1(grid), dim3(256), 0, st, L.insts, tl);
+    hipLaunchKernelGGL(k_lower_inner, g1(grid), dim3(256), 0, st, L.insts, tl, L.marks, stamp);
 }
 
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st) {

@@ -12,10 +12,11 @@ void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st);
 void launch_stage_init(const LaunchSet& L, hipStream_t st);
 // `w`: which of the two active lists the scan of this iteration writes
 void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, int w, hipStream_t st);
-void launch_tile_scan(const LaunchSet& L, int w, hipStream_t st);
+// stamp 0 scans every tile; otherwise only tiles marked with `stamp`
+void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
-void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, hipStream_t st);
+void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st);
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st);
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st);
 void launch_sync(const LaunchSet& L, hipStream_t st);

@@ -84,7 +84,8 @@ Engine::~Engine() {
         if (in.block) (void)hipFree(in.block);
         gp_release(in.gpset);
     }
-    void* dev[] = {d_insts_, d_ctl_, d_cfl_, d_dt_, d_pad_, d_band_, d_cell_, d_act_, d_part_, d_stage_, d_stage_i_};
+    void* dev[] = {d_insts_, d_ctl_,  d_cfl_,  d_dt_,    d_pad_,   d_band_,
+                   d_cell_,  d_act_,  d_marks_, d_part_, d_stage_, d_stage_i_};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     void* host[] = {h_ctl_, h_cfl_, h_dt_, h_scheme_};
@@ -307,7 +308,7 @@ int Engine::commit() {
     // every tile, the tiles touching a guard band, the interior tiles (CFL)
     std::vector<Item> pad, band, cell;
     std::vector<Inst> table(n);
-    size_t max_pad = 0, sum_pad = 0;
+    size_t max_pad = 0, sum_pad = 0, ntiles = 0;
     for (int R = 0; R <= kMaxR; R++) has_R_[R] = false;
     bool any_periodic = false;
     for (size_t k = 0; k < n; k++) {
@@ -319,6 +320,9 @@ int Engine::commit() {
         d.v0 = h.v0_valid ? 1 : 0;
         d.cell_off = (int32_t)cell.size();
         d.cell_cnt = 0;
+        d.ntx = (h.mx + kCellT - 1) / kCellT;
+        d.tile_off = (int32_t)ntiles;
+        ntiles += (size_t)d.ntx * ((h.my + kCellT - 1) / kCellT);
         const size_t np = (size_t)h.mx * h.my;
         max_pad = std::max(max_pad, np);
         sum_pad += np;
@@ -354,6 +358,10 @@ int Engine::commit() {
     if ((rc = put(band, &d_band_, cap_band_))) return rc;
     if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
     if ((rc = grow(&d_act_, cap_act_, 2 * (pad.size() + 1), false))) return rc;
+    if ((rc = grow(&d_marks_, cap_marks_, ntiles + 1, false))) return rc;
+    if ((rc = hip_check(hipMemsetAsync(d_marks_, 0, sizeof(uint32_t) * (ntiles + 1), st_), "hipMemsetAsync(marks)")))
+        return rc;
+    stamp_ = 0;
     if ((rc = grow(&d_part_, cap_part_, 2 * cell.size() + 2, false))) return rc;
     if ((rc = grow(&d_stage_, cap_stage_, 10 * max_pad, false))) return rc;
     if ((rc = grow(&d_stage_i_, cap_stage_i_, max_pad, false))) return rc;
@@ -368,6 +376,7 @@ int Engine::commit() {
     L.act[0] = d_act_;
     L.act[1] = d_act_ + pad.size() + 1;
     L.act_count = d_ctl_ + (size_t)CTL_N * n;
+    L.marks = d_marks_;
     est_act_ = L.npad;
     L.cflpart = d_part_;
     ctl_.assign((size_t)CTL_N * n, 0);
@@ -482,9 +491,11 @@ void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
 int Engine::enqueue_iteration(int s, int it, int grid) {
     const int n = (int)inst_.size(), r = (it - 1) & 1, w = it & 1;
     const TileList act{L.act[r], 0, L.act_count + r};
+    // candidate stamps are never 0 (0 asks the scan for every tile)
+    if (++stamp_ == 0) stamp_ = 1;
     launch_ctl(L, n, OP_NEXT, s, w, st_);
     prof_begin(PF_RECON);
-    launch_lower_inner(L, act, grid, st_);
+    launch_lower_inner(L, act, grid, stamp_, st_);
     prof_end(PF_RECON);
     prof_begin(PF_FACE_FLUX);
     launch_face_groups(act, grid, false, s);
@@ -495,7 +506,7 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, act, grid, st_);
     launch_sync(L, st_);
-    launch_tile_scan(L, w, st_);
+    launch_tile_scan(L, w, stamp_, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
@@ -523,7 +534,7 @@ int Engine::run_stage(int s) {
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, all, L.npad, st_);
     launch_sync(L, st_);
-    launch_tile_scan(L, 0, st_);
+    launch_tile_scan(L, 0, 0, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;

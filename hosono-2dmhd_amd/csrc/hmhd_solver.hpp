@@ -135,6 +135,9 @@ class Engine {
     size_t cap_cell_ = 0;
     Item* d_act_ = nullptr;  // the two active lists
     size_t cap_act_ = 0;
+    uint32_t* d_marks_ = nullptr;  // scan candidates, one word per tile
+    size_t cap_marks_ = 0;
+    uint32_t stamp_ = 0;           // last candidate stamp handed out
     unsigned long long* d_part_ = nullptr;
     size_t cap_part_ = 0;
     uint8_t* h_scheme_ = nullptr;
