@@ -96,8 +96,8 @@ struct LaunchSet {
     const Inst* insts;
     const Item* pad;   // every tile of the padded grids
     int npad;
-    const Item* band;  // tiles touching a guard band (periodic sync)
-    int nband;
+    int ninst;
+    int sync_gx;       // blocks per instance of the periodic MOOD sync (0: no periodic solve)
     const Item* cell;  // interior tiles for CFL
     int ncell;
     Item* act[2];        // active tiles, written by alternate MOOD iterations

@@ -600,8 +600,15 @@ __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ i
     for (int b = blockIdx.x; b < n; b += gridDim.x) lower_detect_tile(insts, tl.items[b]);
 }
 
-// periodic MOOD sync: x sweep over every row, then y sweep over every column
-// (src/boundaryConditions.f90:205-252); needs N >= 2 NGC so pairs are disjoint
+// Periodic MOOD sync (src/boundaryConditions.f90:205-252) in one launch.
+// The reference sweeps x over every row, then y over every column, pairing
+// each guard cell with its interior image N cells away (OR of rc and rcF,
+// MIN of scheme).  The pairs fall into independent units: a guard column's
+// pair in a row that is neither a guard row nor a guard row's image (x
+// only), a guard row's pair in such a column (y only), and the quad of a
+// guard column and a guard row with their images, whose two x pairs come
+// before its two y pairs as in the sweeps.  Needs N >= 2 NGC, so the guard
+// and image bands are disjoint.  One grid row of blocks per instance.
 __device__ __forceinline__ void sync_pair(const Inst& I, size_t a, size_t b) {
     uint8_t e = I.rcF[a] | I.rcF[b];
     I.rcF[a] = e;
@@ -615,28 +622,42 @@ __device__ __forceinline__ void sync_pair(const Inst& I, size_t a, size_t b) {
     I.scheme[b] = m;
 }
 
-__global__ __launch_bounds__(256) void k_sync_x(const Inst* __restrict__ insts, const Item* __restrict__ items) {
-    const Item it = items[blockIdx.x];
-    const Inst& I = insts[it.inst];
-    if (I.bc != BC_PERIODIC || !pass_ready(I)) return;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
-    if (i >= I.mx || j >= I.my) return;
-    const int s = I.ngc, e = I.ngc + I.nx - 1;
-    if (i >= s && i <= e) return;
-    const int ip = i < s ? i + I.nx : i - I.nx;
-    sync_pair(I, (size_t)j * I.mx + i, (size_t)j * I.mx + ip);
-}
-
-__global__ __launch_bounds__(256) void k_sync_y(const Inst* __restrict__ insts, const Item* __restrict__ items) {
-    const Item it = items[blockIdx.x];
-    const Inst& I = insts[it.inst];
-    if (I.bc != BC_PERIODIC || !pass_ready(I)) return;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
-    if (i >= I.mx || j >= I.my) return;
-    const int s = I.ngc, e = I.ngc + I.ny - 1;
-    if (j >= s && j <= e) return;
-    const int jp = j < s ? j + I.ny : j - I.ny;
-    sync_pair(I, (size_t)j * I.mx + i, (size_t)jp * I.mx + i);
+__global__ __launch_bounds__(256) void k_sync(const Inst* __restrict__ insts) {
+    const Inst& I = insts[blockIdx.y];
+    if (I.bc != BC_PERIODIC || !I.active || !pass_ready(I)) return;
+    const int s = I.ngc, g2 = 2 * s, nx = I.nx, ny = I.ny;
+    const size_t mx = (size_t)I.mx;
+    const int nq = g2 * g2, nxo = g2 * (ny - g2), nyo = g2 * (nx - g2);
+    // guard column / row index g in [0, 2s) -> its padded index and image
+    auto col = [&](int g, int& i, int& ip) {
+        i = g < s ? g : nx + g;
+        ip = g < s ? i + nx : i - nx;
+    };
+    auto row = [&](int g, int& j, int& jp) {
+        j = g < s ? g : ny + g;
+        jp = g < s ? j + ny : j - ny;
+    };
+    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < nq + nxo + nyo; u += gridDim.x * blockDim.x) {
+        int i, ip, j, jp;
+        if (u < nq) {
+            col(u % g2, i, ip);
+            row(u / g2, j, jp);
+            sync_pair(I, j * mx + i, j * mx + ip);
+            sync_pair(I, jp * mx + i, jp * mx + ip);
+            sync_pair(I, j * mx + i, jp * mx + i);
+            sync_pair(I, j * mx + ip, jp * mx + ip);
+        } else if (u < nq + nxo) {
+            const int v = u - nq;
+            col(v % g2, i, ip);
+            const size_t r = (size_t)(g2 + v / g2);
+            sync_pair(I, r * mx + i, r * mx + ip);
+        } else {
+            const int v = u - nq - nxo;
+            row(v % g2, j, jp);
+            const size_t c = (size_t)(g2 + v / g2);
+            sync_pair(I, j * mx + c, jp * mx + c);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ //
@@ -1128,9 +1149,8 @@ void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t 
 }
 
 void launch_sync(const LaunchSet& L, hipStream_t st) {
-    if (!L.nband) return;
-    hipLaunchKernelGGL(k_sync_x, g1(L.nband), dim3(256), 0, st, L.insts, L.band);
-    hipLaunchKernelGGL(k_sync_y, g1(L.nband), dim3(256), 0, st, L.insts, L.band);
+    if (L.sync_gx <= 0) return;
+    hipLaunchKernelGGL(k_sync, dim3(L.sync_gx, L.ninst), dim3(256), 0, st, L.insts);
 }
 
 void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st) {

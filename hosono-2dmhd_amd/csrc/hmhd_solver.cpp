@@ -84,8 +84,8 @@ Engine::~Engine() {
         if (in.block) (void)hipFree(in.block);
         gp_release(in.gpset);
     }
-    void* dev[] = {d_insts_, d_ctl_,  d_cfl_,  d_dt_,    d_pad_,   d_band_,
-                   d_cell_,  d_act_,  d_marks_, d_part_, d_stage_, d_stage_i_};
+    void* dev[] = {d_insts_, d_ctl_,   d_cfl_,  d_dt_,    d_pad_,    d_cell_,
+                   d_act_,   d_marks_, d_part_, d_stage_, d_stage_i_};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     void* host[] = {h_ctl_, h_cfl_, h_dt_, h_scheme_};
@@ -306,11 +306,11 @@ int Engine::commit() {
 
     // tiles of the active solves, all 16 x 16 tiles of each padded grid:
     // every tile, the tiles touching a guard band, the interior tiles (CFL)
-    std::vector<Item> pad, band, cell;
+    std::vector<Item> pad, cell;
+    int max_sync_units = 0;
     std::vector<Inst> table(n);
     size_t max_pad = 0, sum_pad = 0, ntiles = 0;
     for (int R = 0; R <= kMaxR; R++) has_R_[R] = false;
-    bool any_periodic = false;
     for (size_t k = 0; k < n; k++) {
         InstHost& h = inst_[k];
         Inst d = h.dev;
@@ -332,15 +332,10 @@ int Engine::commit() {
             tiles(cell, (int)k, h.cfg.nx, h.cfg.ny);
             d.cell_cnt = (int32_t)cell.size() - d.cell_off;
             if (h.cfg.bc == HMHD_BC_PERIODIC) {
-                any_periodic = true;
-                const int ntx = (h.mx + kCellT - 1) / kCellT, nty = (h.my + kCellT - 1) / kCellT;
-                const int ex = h.ngc + h.cfg.nx - 1, ey = h.ngc + h.cfg.ny - 1;
-                for (int ty = 0; ty < nty; ty++)
-                    for (int tx = 0; tx < ntx; tx++) {
-                        const int x0 = tx * kCellT, x1 = x0 + kCellT - 1, y0 = ty * kCellT, y1 = y0 + kCellT - 1;
-                        if (x0 < h.ngc || x1 > ex || y0 < h.ngc || y1 > ey)
-                            band.push_back(Item{(int32_t)k, (int16_t)tx, (int16_t)ty});
-                    }
+                // units of k_sync: guard quads, x-only and y-only pairs
+                const int g2 = 2 * h.ngc;
+                const int units = g2 * g2 + g2 * (h.cfg.ny - g2) + g2 * (h.cfg.nx - g2);
+                max_sync_units = std::max(max_sync_units, units);
             }
         }
         h.dev = d;
@@ -355,7 +350,6 @@ int Engine::commit() {
         return hip_check(hipMemcpy(*d, v.data(), sizeof(Item) * v.size(), hipMemcpyHostToDevice), "hipMemcpy(items)");
     };
     if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
-    if ((rc = put(band, &d_band_, cap_band_))) return rc;
     if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
     if ((rc = grow(&d_act_, cap_act_, 2 * (pad.size() + 1), false))) return rc;
     if ((rc = grow(&d_marks_, cap_marks_, ntiles + 1, false))) return rc;
@@ -369,8 +363,8 @@ int Engine::commit() {
     L.insts = d_insts_;
     L.pad = d_pad_;
     L.npad = (int)pad.size();
-    L.band = d_band_;
-    L.nband = any_periodic ? (int)band.size() : 0;
+    L.ninst = (int)n;
+    L.sync_gx = std::min((max_sync_units + 255) / 256, 256);
     L.cell = d_cell_;
     L.ncell = (int)cell.size();
     L.act[0] = d_act_;

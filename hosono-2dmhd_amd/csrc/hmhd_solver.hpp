@@ -129,8 +129,6 @@ class Engine {
     size_t cap_hdt_ = 0;
     Item* d_pad_ = nullptr;
     size_t cap_pad_ = 0;
-    Item* d_band_ = nullptr;
-    size_t cap_band_ = 0;
     Item* d_cell_ = nullptr;
     size_t cap_cell_ = 0;
     Item* d_act_ = nullptr;  // the two active lists
