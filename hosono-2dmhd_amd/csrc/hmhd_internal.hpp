@@ -24,8 +24,8 @@ namespace hmhd {
 
 enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
 enum : int { NCONS = 8 };
-// ctl words: flags the kernels raise, the MOOD-loop state the host steers
-// with k_ctl, and per-stage pass / inner-iteration counters
+// ctl words: flags the kernels raise, the MOOD-loop state k_begin and
+// k_finish keep on the device, and per-stage pass / inner-iteration counters
 enum : int {
     CTL_ANY_INADM = 0,  // an inadmissible reconstructed state was found
     CTL_ANY_LOWER = 1,  // detect lowered a cell
@@ -36,7 +36,6 @@ enum : int {
     CTL_N = 16
 };
 enum : int { MODE_NEWPASS = 0, MODE_INNER = 1 };
-enum : int { OP_BEGIN = 0, OP_NEXT = 1 };
 
 constexpr int kMaxR = 3;
 constexpr int kMaxGpSets = 8;
@@ -94,6 +93,7 @@ struct TileList {
 
 struct LaunchSet {
     const Inst* insts;
+    int32_t* ctl;      // CTL_N words per instance, then the two active-list counts
     const Item* pad;   // every tile of the padded grids
     int npad;
     int ninst;

@@ -850,25 +850,38 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     for (int b = blockIdx.x; b < n; b += gridDim.x) update_tile(insts, tl.items[b], stage);
 }
 
-// MOOD-loop control between passes, one thread per instance.
-// OP_BEGIN starts a stage; OP_NEXT decides, from the flags of the pass just
-// run, whether the instance continues its inner admissibility loop
-// (MODE_INNER), starts a new MOOD pass (MODE_NEWPASS) or leaves the loop.
-__global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int stage, int32_t* __restrict__ scan_count) {
+// Start of a stage's MOOD loop, one thread per instance (inactive
+// instances stay out of the loop); also empties the active list the first
+// pass's scan writes.
+__global__ void k_begin(const Inst* __restrict__ insts, int ninst, int stage, int32_t* __restrict__ scan_count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) *scan_count = 0;
     if (k >= ninst) return;
     int32_t* c = insts[k].ctl;
     if (!insts[k].active) {
         c[CTL_RUN] = 0;
-    } else if (op == OP_BEGIN) {
-        c[CTL_ANY_INADM] = 0;
-        c[CTL_ANY_LOWER] = 0;
-        c[CTL_RUN] = 1;
-        c[CTL_MODE] = MODE_NEWPASS;
-        c[CTL_PASSES + stage] = 1;
-        c[CTL_INNER + stage] = 1;
-    } else if (c[CTL_RUN]) {
+        return;
+    }
+    c[CTL_ANY_INADM] = 0;
+    c[CTL_ANY_LOWER] = 0;
+    c[CTL_RUN] = 1;
+    c[CTL_MODE] = MODE_NEWPASS;
+    c[CTL_PASSES + stage] = 1;
+    c[CTL_INNER + stage] = 1;
+}
+
+// End of a MOOD iteration, one block.  Each instance still in its loop
+// decides from the flags of the iteration just run whether it continues
+// the inner admissibility loop (MODE_INNER), starts a new MOOD pass
+// (MODE_NEWPASS) or leaves the loop (src/timeStep.f90:86-128 with
+// src/reconstruct.f90:146-171); the active list the next iteration's scan
+// writes is emptied; then every control word and both list counts are
+// copied to host memory, where the host tests RUN.
+__global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, int ninst, int stage,
+                                                int32_t* __restrict__ ctl, int w, int32_t* __restrict__ snap) {
+    for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+        int32_t* c = insts[k].ctl;
+        if (!c[CTL_RUN]) continue;
         if (c[CTL_ANY_INADM]) {
             c[CTL_ANY_INADM] = 0;
             c[CTL_MODE] = MODE_INNER;
@@ -882,6 +895,9 @@ __global__ void k_ctl(const Inst* __restrict__ insts, int ninst, int op, int sta
             c[CTL_RUN] = 0;
         }
     }
+    if (threadIdx.x == 0) ctl[CTL_N * ninst + (1 - w)] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < CTL_N * ninst + 2; t += blockDim.x) snap[t] = ctl[t];
 }
 
 // Active tiles of the next MOOD iteration: every quantity a pass writes
@@ -1105,8 +1121,12 @@ void launch_stage_init(const LaunchSet& L, hipStream_t st) {
     hipLaunchKernelGGL(k_stage_init, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
 }
 
-void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, int w, hipStream_t st) {
-    hipLaunchKernelGGL(k_ctl, dim3((ninst + 63) / 64), dim3(64), 0, st, L.insts, ninst, op, stage, L.act_count + w);
+void launch_begin(const LaunchSet& L, int stage, hipStream_t st) {
+    hipLaunchKernelGGL(k_begin, dim3((L.ninst + 63) / 64), dim3(64), 0, st, L.insts, L.ninst, stage, L.act_count);
+}
+
+void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, L.insts, L.ninst, stage, L.ctl, w, snap);
 }
 
 void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st) {

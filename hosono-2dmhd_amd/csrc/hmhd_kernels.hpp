@@ -10,8 +10,12 @@ namespace hmhd {
 
 void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st);
 void launch_stage_init(const LaunchSet& L, hipStream_t st);
+// start of a stage's MOOD loop (the first pass's scan writes list 0)
+void launch_begin(const LaunchSet& L, int stage, hipStream_t st);
+// end of a MOOD iteration whose scan wrote list w: loop decisions, and the
+// control words copied to `snap` (a device address of pinned host memory)
+void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStream_t st);
 // `w`: which of the two active lists the scan of this iteration writes
-void launch_ctl(const LaunchSet& L, int ninst, int op, int stage, int w, hipStream_t st);
 // stamp 0 scans every tile; otherwise only tiles marked with `stamp`
 void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st);
 // tile kernels: `grid` blocks stride over the list

@@ -298,6 +298,9 @@ int Engine::commit() {
     if ((rc = grow(&d_insts_, cap_insts_, n, false))) return rc;
     if ((rc = grow(&d_ctl_, cap_ctl_, nctl, false))) return rc;
     if ((rc = grow(&h_ctl_, cap_hctl_, 2 * nctl, true))) return rc;
+    if ((rc = hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_hctl_), h_ctl_, 0),
+                        "hipHostGetDevicePointer")))
+        return rc;
     if ((rc = grow(&d_cfl_, cap_cfl_, 2 * n, false))) return rc;
     if ((rc = grow(&h_cfl_, cap_hcfl_, 2 * n, true))) return rc;
     if ((rc = grow(&d_dt_, cap_dt_, n, false))) return rc;
@@ -361,6 +364,7 @@ int Engine::commit() {
     if ((rc = grow(&d_stage_i_, cap_stage_i_, max_pad, false))) return rc;
     if ((rc = grow(&h_scheme_, cap_hscheme_, sum_pad, true))) return rc;
     L.insts = d_insts_;
+    L.ctl = d_ctl_;
     L.pad = d_pad_;
     L.npad = (int)pad.size();
     L.ninst = (int)n;
@@ -446,12 +450,13 @@ void Engine::prof_collect(bool wait_all) {
 
 // ------------------------------------------------------------------ //
 
-// copy the control words and both active-list counts to host slot w once
-// the work enqueued so far has run
-int Engine::snapshot(int w) {
+// end of the MOOD iteration whose scan wrote list w in stage s: the loop
+// decisions, and the control words and both list counts written to host
+// slot w, ready when event w has fired
+int Engine::snapshot(int s, int w) {
     const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 2;
-    int rc = hip_check(hipMemcpyAsync(h_ctl_ + w * nctl, d_ctl_, sizeof(int32_t) * nctl, hipMemcpyDeviceToHost, st_),
-                       "hipMemcpyAsync(ctl)");
+    launch_finish(L, s, w, d_hctl_ + w * nctl, st_);
+    int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
     return hip_check(hipEventRecord(ev_snap_[w], st_), "hipEventRecord");
 }
@@ -483,11 +488,10 @@ void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
 // it - 1 (list (it - 1) & 1), its own scan writing list it & 1, then a
 // snapshot into host slot it & 1.  `grid` blocks stride over the list.
 int Engine::enqueue_iteration(int s, int it, int grid) {
-    const int n = (int)inst_.size(), r = (it - 1) & 1, w = it & 1;
+    const int r = (it - 1) & 1, w = it & 1;
     const TileList act{L.act[r], 0, L.act_count + r};
     // candidate stamps are never 0 (0 asks the scan for every tile)
     if (++stamp_ == 0) stamp_ = 1;
-    launch_ctl(L, n, OP_NEXT, s, w, st_);
     prof_begin(PF_RECON);
     launch_lower_inner(L, act, grid, stamp_, st_);
     prof_end(PF_RECON);
@@ -504,20 +508,20 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    return snapshot(w);
+    return snapshot(s, w);
 }
 
 // One RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active
 // solve.  The loop runs one iteration ahead of the host: iteration it + 1
 // is enqueued before the host has read whether iteration it left work.  A
-// speculative iteration after the last needed one is a no-op (k_ctl ends
-// every solve's loop when no flag is set, and every kernel then returns),
-// so the results and the pass counts are those of the plain loop.
+// speculative iteration after the last needed one is a no-op (k_finish has
+// ended every solve's loop, and every kernel then returns), so the results
+// and the pass counts are those of the plain loop.
 int Engine::run_stage(int s) {
     const int n = (int)inst_.size();
     const TileList all{L.pad, L.npad, nullptr};
     // iteration 0, the first pass: every tile, every face at the top radius
-    launch_ctl(L, n, OP_BEGIN, s, 0, st_);
+    launch_begin(L, s, st_);
     launch_stage_init(L, st_);
     prof_begin(PF_FACE_FULL);
     launch_face_groups(all, L.npad, true, s);
@@ -532,7 +536,7 @@ int Engine::run_stage(int s) {
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    if ((rc = snapshot(0))) return rc;
+    if ((rc = snapshot(s, 0))) return rc;
     auto grid_for = [&](int listed) { return std::min(L.npad, listed + listed / 2 + 16); };
     if ((rc = enqueue_iteration(s, 1, grid_for(est_act_)))) return rc;
     for (int it = 0;; it++) {
@@ -543,11 +547,11 @@ int Engine::run_stage(int s) {
             err = "MOOD loop did not converge";
             return HMHD_ERR_DEVICE;
         }
+        // the snapshot is taken after the loop decisions: RUN set means
+        // the solve goes on to iteration it + 1
         bool more = false;
-        for (int k = 0; k < n; k++) {
-            const int32_t* c = &ctl_[(size_t)CTL_N * k];
-            if (c[CTL_RUN] && (c[CTL_ANY_INADM] || c[CTL_ANY_LOWER])) more = true;
-        }
+        for (int k = 0; k < n; k++)
+            if (ctl_[(size_t)CTL_N * k + CTL_RUN]) more = true;
         if (it == 0) est_act_ = act_n_;
         if (!more) break;  // iteration it + 1, already enqueued, does nothing
         // iteration it + 2 reads the list iteration it + 1 writes; size its

@@ -93,7 +93,7 @@ class Engine {
     int hip_check(hipError_t e, const char* what);
     int ensure_stream();
     int commit();
-    int snapshot(int w);
+    int snapshot(int s, int w);
     int read_snapshot(int w);
     int enqueue_iteration(int s, int it, int grid);
     int run_stage(int s);
@@ -117,6 +117,7 @@ class Engine {
     int32_t* d_ctl_ = nullptr;  // CTL_N words per solve, then the active-tile count
     size_t cap_ctl_ = 0;
     int32_t* h_ctl_ = nullptr;  // pinned, two snapshot slots
+    int32_t* d_hctl_ = nullptr;  // its device address
     size_t cap_hctl_ = 0;
     hipEvent_t ev_snap_[2] = {nullptr, nullptr};
     unsigned long long* d_cfl_ = nullptr;  // two maxima per solve
