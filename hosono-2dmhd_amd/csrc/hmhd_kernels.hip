@@ -139,6 +139,20 @@ __device__ __forceinline__ bool pass_ready(const Inst& I) {
     return I.ctl[CTL_RUN] != 0 && I.ctl[CTL_ANY_INADM] == 0;
 }
 
+// Workgroups go to the 8 XCDs round-robin, and each XCD has its own L2.
+// When a grid has one block per tile of a row-major list, block b takes
+// tile xcd_tile(b, n): every XCD gets one contiguous run of the list (a
+// band of tile rows), so neighbouring tiles share their halo lines in one
+// L2 instead of fetching them on several XCDs.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    constexpr int kXcd = 8;
+    const int q = n / kXcd, r = n % kXcd, x = b % kXcd;
+    return x * q + (x < r ? x : r) + b / kXcd;
+}
+
+// tile of loop step b: swizzled when the grid covers the list once
+__device__ __forceinline__ int list_tile(int b, int n) { return (int)gridDim.x >= n ? xcd_tile(b, n) : b; }
+
 // SSP-RK3 buffer roles from the solve's rotation a = I.rot (U^n in U[a]):
 // stage 0 reads U[a] and writes U1 to U[a+1]; stage 1 reads U[a+1] and
 // writes U2 to U[a+2]; stage 2 reads U[a+2] and writes U^{n+1} to U[a+1];
@@ -175,7 +189,7 @@ __device__ __forceinline__ void accumulate_flux(double* acc, const Phys& ph, con
 // stage set-up: scheme = R, rc = rcF = true everywhere
 // (src/timeStep.f90:79-81); inadm cleared; every face at radius R
 __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ insts, const Item* __restrict__ items) {
-    const Item it = items[blockIdx.x];
+    const Item it = items[xcd_tile(blockIdx.x, gridDim.x)];
     const Inst& I = insts[it.inst];
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i >= I.mx || j >= I.my) return;
@@ -485,10 +499,11 @@ __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES
     const int n = tl.cnt ? *tl.cnt : tl.n;
     for (int b = blockIdx.x; b < n; b += gridDim.x) {
         if (b != (int)blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
+        const Item it = tl.items[list_tile(b, n)];
         if (FULL)
-            face_tile_full<R>(insts, tl.items[b], stage, sU);
+            face_tile_full<R>(insts, it, stage, sU);
         else
-            face_tile_iter<R>(insts, tl.items[b], stage, sU, lst, lcnt);
+            face_tile_iter<R>(insts, it, stage, sU, lst, lcnt);
     }
 }
 
@@ -576,7 +591,7 @@ __device__ __forceinline__ void lower_inner_tile(const Inst* __restrict__ insts,
 __global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const TileList tl,
                                                      uint32_t* __restrict__ marks, uint32_t stamp) {
     const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_inner_tile(insts, tl.items[b], marks, stamp);
+    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_inner_tile(insts, tl.items[list_tile(b, n)], marks, stamp);
 }
 
 // after detect: rc = cross(low), rcF = diamond(low), scheme lowered
@@ -597,7 +612,7 @@ __device__ __forceinline__ void lower_detect_tile(const Inst* __restrict__ insts
 
 __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const TileList tl) {
     const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_detect_tile(insts, tl.items[b]);
+    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_detect_tile(insts, tl.items[list_tile(b, n)]);
 }
 
 // Periodic MOOD sync (src/boundaryConditions.f90:205-252) in one launch.
@@ -847,7 +862,7 @@ __device__ __forceinline__ void update_tile(const Inst* __restrict__ insts, cons
 
 __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const TileList tl, int stage) {
     const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) update_tile(insts, tl.items[b], stage);
+    for (int b = blockIdx.x; b < n; b += gridDim.x) update_tile(insts, tl.items[list_tile(b, n)], stage);
 }
 
 // Start of a stage's MOOD loop, one thread per instance (inactive
@@ -908,7 +923,7 @@ __global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, 
 __global__ __launch_bounds__(256) void k_tile_scan(const Inst* __restrict__ insts, const Item* __restrict__ items,
                                                    Item* __restrict__ act, int32_t* __restrict__ act_count,
                                                    const uint32_t* __restrict__ marks, uint32_t stamp) {
-    const Item it = items[blockIdx.x];
+    const Item it = items[xcd_tile(blockIdx.x, gridDim.x)];
     const Inst& I = insts[it.inst];
     if (stamp != 0 && marks[I.tile_off + it.ty * I.ntx + it.tx] != stamp) return;
     const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
@@ -937,7 +952,9 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 
 __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, const Item* __restrict__ items,
                                              unsigned long long* __restrict__ part) {
-    const Item it = items[blockIdx.x];
+    // partials are stored per tile, so each instance's lie at [cell_off, cell_off + cell_cnt)
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const Item it = items[t];
     const Inst& I = insts[it.inst];
     const int ki = it.tx * kCellT + (threadIdx.x % kCellT), kj = it.ty * kCellT + (threadIdx.x / kCellT);
     unsigned long long bx = 0, by = 0;
@@ -996,8 +1013,8 @@ __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, con
             bx = wx[w] > bx ? wx[w] : bx;
             by = wy[w] > by ? wy[w] : by;
         }
-        part[2 * blockIdx.x] = bx;
-        part[2 * blockIdx.x + 1] = by;
+        part[2 * t] = bx;
+        part[2 * t + 1] = by;
     }
 }
 
