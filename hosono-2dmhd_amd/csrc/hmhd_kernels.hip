@@ -681,27 +681,6 @@ __global__ __launch_bounds__(256) void k_sync(const Inst* __restrict__ insts) {
 // stage 0: out = Unew; stage 1: out = (3 base + Unew)/4;
 // stage 2: out = (base + 2 Unew)/3.
 
-struct FaceRead {
-    const double* FR;
-    size_t fp;
-    int mx;
-    __device__ __forceinline__ double x(int v, int i, int j) const { return FR[v * fp + (size_t)j * mx + i]; }
-    __device__ __forceinline__ double y(int v, int i, int j) const {
-        return FR[(NCONS + v) * fp + (size_t)j * mx + i];
-    }
-    // src/FLICDdivB.f90:38-45
-    __device__ __forceinline__ double corr_x(int i, int j) const {
-        return (2.0 * x(MAGY, i, j) + x(MAGY, i + 1, j) + x(MAGY, i - 1, j) - y(MAGX, i, j + 1) - y(MAGX, i, j) -
-                y(MAGX, i - 1, j) - y(MAGX, i - 1, j + 1)) /
-               8.0;
-    }
-    // src/FLICDdivB.f90:47-54
-    __device__ __forceinline__ double corr_y(int i, int j) const {
-        return (2.0 * y(MAGX, i, j) + y(MAGX, i, j + 1) + y(MAGX, i, j - 1) - x(MAGY, i + 1, j) - x(MAGY, i, j) -
-                x(MAGY, i, j - 1) - x(MAGY, i + 1, j - 1)) /
-               8.0;
-    }
-};
 
 // NaN-skipping MINVAL / MAXVAL of five values
 __device__ __forceinline__ double minval5(const double* x) {
@@ -719,36 +698,62 @@ __device__ __forceinline__ double maxval5(const double* x) {
     return m;
 }
 
-struct CellRead {
-    const double* U;
-    size_t plane;
-    int ngc, nx, ny, bc;
-    __device__ __forceinline__ size_t off(int i, int j) const {
-        return (size_t)map_axis(j, ngc, ny, bc) * nx + map_axis(i, ngc, nx, bc);
-    }
-    __device__ __forceinline__ void load(int i, int j, double* u) const {
-        const size_t o = off(i, j);
-#pragma unroll
-        for (int v = 0; v < NCONS; v++) u[v] = U[v * plane + o];
-    }
-    __device__ __forceinline__ double dens(int i, int j) const { return U[off(i, j)]; }
+// The values around one cell that detection reads, loaded together
+// before any is used: U of the cross neighbours and the density of the
+// cells at taxicab distance 2 (guard cells through the index map).
+struct Around {
+    double e[NCONS], w[NCONS], n[NCONS], s[NCONS];  // U at (i+1,j), (i-1,j), (i,j+1), (i,j-1)
+    double d2[8];  // rho at (i+2,j), (i-2,j), (i,j+2), (i,j-2), (i+1,j+1), (i+1,j-1), (i-1,j+1), (i-1,j-1)
 };
 
+__device__ __forceinline__ void load_around(const Inst& I, const double* __restrict__ U, int i, int j, Around& a) {
+    const size_t plane = (size_t)I.nx * I.ny;
+    auto off = [&](int di, int dj) {
+        return (size_t)map_axis(j + dj, I.ngc, I.ny, I.bc) * I.nx + map_axis(i + di, I.ngc, I.nx, I.bc);
+    };
+    const size_t oe = off(1, 0), ow = off(-1, 0), on = off(0, 1), os = off(0, -1);
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) {
+        a.e[v] = U[v * plane + oe];
+        a.w[v] = U[v * plane + ow];
+        a.n[v] = U[v * plane + on];
+        a.s[v] = U[v * plane + os];
+    }
+    a.d2[0] = U[off(2, 0)];
+    a.d2[1] = U[off(-2, 0)];
+    a.d2[2] = U[off(0, 2)];
+    a.d2[3] = U[off(0, -2)];
+    a.d2[4] = U[off(1, 1)];
+    a.d2[5] = U[off(1, -1)];
+    a.d2[6] = U[off(-1, 1)];
+    a.d2[7] = U[off(-1, -1)];
+}
+
+// density at offset (di, dj), |di| + |dj| <= 2, from the centre cell and `a`
+__device__ __forceinline__ double dens_at(const double* uc, const Around& a, int di, int dj) {
+    if (di == 0 && dj == 0) return uc[DENS];
+    if (di == 1 && dj == 0) return a.e[DENS];
+    if (di == -1 && dj == 0) return a.w[DENS];
+    if (di == 0 && dj == 1) return a.n[DENS];
+    if (di == 0 && dj == -1) return a.s[DENS];
+    if (di == 2) return a.d2[0];
+    if (di == -2) return a.d2[1];
+    if (dj == 2) return a.d2[2];
+    if (dj == -2) return a.d2[3];
+    if (di == 1) return dj == 1 ? a.d2[4] : a.d2[5];
+    return dj == 1 ? a.d2[6] : a.d2[7];
+}
+
 // src/detect.f90:83-213 for one checked cell; true = lower the order
-__device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const CellRead& cr, int i, int j,
-                                            const double* un, const double* uc) {
+__device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const Around& a, const double* un,
+                                            const double* uc) {
     const double presNew = cons2pres(ph, un);
     if (!admissible(un[DENS], presNew)) return true;
-    double u[NCONS];
     const Prim V = cons2prim(ph, uc);
-    cr.load(i + 1, j, u);
-    const Prim Vip = cons2prim(ph, u);
-    cr.load(i - 1, j, u);
-    const Prim Vim = cons2prim(ph, u);
-    cr.load(i, j + 1, u);
-    const Prim Vjp = cons2prim(ph, u);
-    cr.load(i, j - 1, u);
-    const Prim Vjm = cons2prim(ph, u);
+    const Prim Vip = cons2prim(ph, a.e);
+    const Prim Vim = cons2prim(ph, a.w);
+    const Prim Vjp = cons2prim(ph, a.n);
+    const Prim Vjm = cons2prim(ph, a.s);
     double div_vel = (Vip.vx - Vim.vx) / (2.0 * I.dx);
     div_vel = div_vel + (Vjp.vy - Vjm.vy) / (2.0 * I.dy);
     const bool shock1 = div_vel < -(1.0 * (I.maxdxdy * I.maxdxdy));
@@ -762,8 +767,7 @@ __device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const
     if (!(shock1 && shock2 && shock3)) return false;
 
     const double densNew = un[DENS];
-    const double d1 = uc[DENS], d2 = cr.dens(i + 1, j), d3 = cr.dens(i - 1, j), d4 = cr.dens(i, j + 1),
-                 d5 = cr.dens(i, j - 1);
+    const double d1 = uc[DENS], d2 = a.e[DENS], d3 = a.w[DENS], d4 = a.n[DENS], d5 = a.s[DENS];
     const double mn = fminF(fminF(fminF(fminF(d1, d2), d3), d4), d5);
     const double mxd = fmaxF(fmaxF(fmaxF(fmaxF(d1, d2), d3), d4), d5);
     if (mxd - mn < I.mindxdy * I.mindxdy * I.mindxdy) return false;
@@ -772,13 +776,12 @@ __device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const
     // u2 smoothness on the five cells of the cross (src/detect.f90:177-213)
     double ddx[5], ddy[5];
     const double dx2 = I.dx * I.dx, dy2 = I.dy * I.dy;
-    const int ci[5] = {-1, 0, 0, 0, 1}, cj[5] = {0, -1, 0, 1, 0};
+    constexpr int ci[5] = {-1, 0, 0, 0, 1}, cj[5] = {0, -1, 0, 1, 0};
 #pragma unroll
     for (int c = 0; c < 5; c++) {
-        const int ii = i + ci[c], jj = j + cj[c];
-        const double dcc = cr.dens(ii, jj);
-        ddx[c] = (cr.dens(ii + 1, jj) - 2.0 * dcc + cr.dens(ii - 1, jj)) / dx2;
-        ddy[c] = (cr.dens(ii, jj + 1) - 2.0 * dcc + cr.dens(ii, jj - 1)) / dy2;
+        const double dcc = dens_at(uc, a, ci[c], cj[c]);
+        ddx[c] = (dens_at(uc, a, ci[c] + 1, cj[c]) - 2.0 * dcc + dens_at(uc, a, ci[c] - 1, cj[c])) / dx2;
+        ddy[c] = (dens_at(uc, a, ci[c], cj[c] + 1) - 2.0 * dcc + dens_at(uc, a, ci[c], cj[c] - 1)) / dy2;
     }
     const double mnx = minval5(ddx), mny = minval5(ddy), mxx = maxval5(ddx), mxy = maxval5(ddy);
     const double m = I.mindxdy;
@@ -792,6 +795,12 @@ __device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const
     return !(smooth1 || smooth2);
 }
 
+// One tile of the FV update.  An own (rcF) cell loads everything its
+// update reads -- the FLICD stencil of By on x-faces and Bx on y-faces,
+// the fluxes of its four faces, the neighbours' flags and FLB values, U
+// around it -- before using any of it, so the global loads overlap.  The
+// arithmetic and its order are src/FLICDdivB.f90:38-54 and
+// src/timeStep.f90:107-118 as written.
 __device__ __forceinline__ void update_tile(const Inst* __restrict__ insts, const Item it, int stage) {
     const Inst& I = insts[it.inst];
     if (!pass_ready(I)) return;
@@ -799,49 +808,70 @@ __device__ __forceinline__ void update_tile(const Inst* __restrict__ insts, cons
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i < s || j < s || i > ex + 1 || j > ey + 1) return;
     const size_t idx = (size_t)j * mx + i, fp = (size_t)mx * I.my;
-    const FaceRead fr{I.FR, fp, mx};
-    const bool own = I.rcF[idx];
-    double cx = 0.0, cy = 0.0;
-    if (own) {
-        cx = fr.corr_x(i, j);
-        cy = fr.corr_y(i, j);
-        I.FLB[idx] = cx;
-        I.FLB[fp + idx] = cy;
-    }
-    if (i > ex || j > ey) return;
-    const size_t ci = (size_t)(j - s) * I.nx + (i - s), plane = (size_t)I.nx * I.ny;
-    if (!own) {
-        I.low[idx] = 0;
+    const bool inner = i <= ex && j <= ey;
+    if (!I.rcF[idx]) {
+        if (inner) I.low[idx] = 0;
         return;
     }
-    const double cxe = I.rcF[idx + 1] ? fr.corr_x(i + 1, j) : I.FLB[idx + 1];
-    const double cyn = I.rcF[idx + mx] ? fr.corr_y(i, j + 1) : I.FLB[fp + idx + mx];
-    const double dtdx = I.dtdx, dtdy = I.dtdy;
+    // By on x-faces (x(i+a, j+b) = X[idx + a + b mx]) and Bx on y-faces
+    const double* __restrict__ X = I.FR + MAGY * fp;
+    const double* __restrict__ Y = I.FR + (NCONS + MAGX) * fp;
+    const double xw = X[idx - 1], x0 = X[idx], x1 = X[idx + 1], x2 = X[idx + 2];
+    const double xs0 = X[idx - mx], xs1 = X[idx - mx + 1], xn0 = X[idx + mx], xn1 = X[idx + mx + 1];
+    const double ys = Y[idx - mx], y0 = Y[idx], y1 = Y[idx + mx], y2 = Y[idx + 2 * mx];
+    const double yw0 = Y[idx - 1], yw1 = Y[idx - 1 + mx], ye0 = Y[idx + 1], ye1 = Y[idx + 1 + mx];
+    const uint8_t rcfE = I.rcF[idx + 1], rcfN = I.rcF[idx + mx];
+    const double flbE = I.FLB[idx + 1], flbN = I.FLB[fp + idx + mx];
     const StageBufs sb = stage_bufs(I.rot, stage);
     const double* __restrict__ Uin = I.U[sb.in];
-    double uc[NCONS], un[NCONS];
+    const size_t plane = (size_t)I.nx * I.ny;
+    const size_t ci = inner ? (size_t)(j - s) * I.nx + (i - s) : 0;
+    double fxm[NCONS], fxp[NCONS], fym[NCONS], fyp[NCONS], uc[NCONS], ub[NCONS];
+    Around a;
+    const int sch = inner ? I.scheme[idx] : 0;
+    if (inner) {
 #pragma unroll
-    for (int v = 0; v < NCONS; v++) {
-        double fxm = fr.x(v, i, j), fxp = fr.x(v, i + 1, j), fym = fr.y(v, i, j), fyp = fr.y(v, i, j + 1);
-        if (v == MAGX) {
-            fxm = 0.0;
-            fxp = 0.0;
-            fym = cy;
-            fyp = cyn;
-        } else if (v == MAGY) {
-            fxm = cx;
-            fxp = cxe;
-            fym = 0.0;
-            fyp = 0.0;
+        for (int v = 0; v < NCONS; v++) {
+            if (v == MAGX || v == MAGY) continue;
+            fxm[v] = I.FR[v * fp + idx];
+            fxp[v] = I.FR[v * fp + idx + 1];
+            fym[v] = I.FR[(NCONS + v) * fp + idx];
+            fyp[v] = I.FR[(NCONS + v) * fp + idx + mx];
         }
-        uc[v] = Uin[v * plane + ci];
-        un[v] = uc[v] - dtdx * (fxp - fxm) - dtdy * (fyp - fym);
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) uc[v] = Uin[v * plane + ci];
+        if (stage > 0) {
+            const double* __restrict__ Ub = I.U[sb.base];
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) ub[v] = Ub[v * plane + ci];
+        }
+        load_around(I, Uin, i, j, a);
     }
+    // src/FLICDdivB.f90:38-45 and :47-54 at (i, j)
+    const double cx = (2.0 * x0 + x1 + xw - y1 - y0 - yw0 - yw1) / 8.0;
+    const double cy = (2.0 * y0 + y1 + ys - x1 - x0 - xs0 - xs1) / 8.0;
+    I.FLB[idx] = cx;
+    I.FLB[fp + idx] = cy;
+    if (!inner) return;
+    // the east / north neighbour's correction, or the value its face holds
+    const double cxe = rcfE ? (2.0 * x1 + x2 + x0 - ye1 - ye0 - y0 - y1) / 8.0 : flbE;
+    const double cyn = rcfN ? (2.0 * y1 + y2 + y0 - xn1 - xn0 - x0 - x1) / 8.0 : flbN;
+    fxm[MAGX] = 0.0;
+    fxp[MAGX] = 0.0;
+    fym[MAGX] = cy;
+    fyp[MAGX] = cyn;
+    fxm[MAGY] = cx;
+    fxp[MAGY] = cxe;
+    fym[MAGY] = 0.0;
+    fyp[MAGY] = 0.0;
+    const double dtdx = I.dtdx, dtdy = I.dtdy;
+    double un[NCONS];
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) un[v] = uc[v] - dtdx * (fxp[v] - fxm[v]) - dtdy * (fyp[v] - fym[v]);
     bool lower = false;
-    if (I.scheme[idx] > 0) {
+    if (sch > 0) {
         const Phys ph = phys_of(I);
-        const CellRead cr{Uin, plane, s, I.nx, I.ny, I.bc};
-        lower = detect_cell(I, ph, cr, i, j, un, uc);
+        lower = detect_cell(I, ph, a, un, uc);
     }
     I.low[idx] = lower ? 1 : 0;
     if (lower) I.ctl[CTL_ANY_LOWER] = 1;
@@ -850,13 +880,11 @@ __device__ __forceinline__ void update_tile(const Inst* __restrict__ insts, cons
 #pragma unroll
         for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = un[v];
     } else if (stage == 1) {
-        const double* __restrict__ Ub = I.U[sb.base];
 #pragma unroll
-        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (3.0 * Ub[v * plane + ci] + un[v]) / 4.0;
+        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (3.0 * ub[v] + un[v]) / 4.0;
     } else {
-        const double* __restrict__ Ub = I.U[sb.base];
 #pragma unroll
-        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (Ub[v * plane + ci] + 2.0 * un[v]) / 3.0;
+        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (ub[v] + 2.0 * un[v]) / 3.0;
     }
 }
 
