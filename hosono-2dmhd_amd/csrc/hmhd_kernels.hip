@@ -205,34 +205,6 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
     }
 }
 
-// ------------------------------------------------------------------ //
-// One face after the first pass of a stage (src/reconstruct.f90:77-141 and
-// src/getFluxes.f90:26-88): a side whose cell is flagged for
-// reconstruction is rebuilt at the current EDP1 radius rr_now and its states
-// checked (CAD/PAD); any other side keeps the radius it was last
-// reconstructed with; the flux, when requested, uses rr_now points and
-// weights.
-template <int R, int DIR, class A>
-__device__ __forceinline__ void face_iter(const Phys& ph, int set, const A& aL, const A& aR, int fL, int fR,
-                                          bool recL, bool recR, int rr_now, int rrL_old, int rrR_old, bool flux,
-                                          double* acc, bool& badL, bool& badR) {
-    const int rrL = recL ? rr_now : rrL_old, rrR = recR ? rr_now : rrR_old;
-    const int nqL = (recL && rrL > 0) ? gp_nq(rrL) : 0;
-    const int nqR = (recR && rrR > 0) ? gp_nq(rrR) : 0;
-    const int nqF = flux ? (rr_now == 0 ? 1 : gp_nq(rr_now)) : 0;
-    int nq = nqL > nqR ? nqL : nqR;
-    nq = nqF > nq ? nqF : nq;
-    badL = badR = false;
-    for (int q = 0; q < nq; q++) {
-        double uL[NCONS], uR[NCONS];
-        reload_barrier();
-        if (q < nqL || q < nqF) side_value<R>(aL, set, rrL, fL, q, uL);
-        if (q < nqR || q < nqF) side_value<R>(aR, set, rrR, fR, q, uR);
-        if (q < nqL) badL |= !admissible(uL[DENS], cons2pres(ph, uL));
-        if (q < nqR) badR |= !admissible(uR[DENS], cons2pres(ph, uR));
-        if (q < nqF) accumulate_flux<DIR>(acc, ph, uL, uR, c_qw[set][rr_now][q], q == 0, rr_now == 0);
-    }
-}
 
 // ------------------------------------------------------------------ //
 // face kernel, one block per 16 x 16 tile (blocks stride over the tile
@@ -361,6 +333,75 @@ __device__ __forceinline__ void face_tile_full(const Inst* __restrict__ insts, c
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
 
+// One face of a later MOOD iteration on the four lanes of a quad, lane q
+// for Gauss point q (src/reconstruct.f90:109-133, src/riemannSolver.f90:
+// 37-44): the lanes rebuild their point of each side to be rebuilt (at
+// the current radius) or needed for the flux (at the radius the side was
+// last rebuilt with), check its admissibility, and form w_q F_HLLC; lane 0
+// sums the points in ascending order from zero (or takes the one
+// first-order flux as it is) and writes the face's results.
+template <int R, int DIR, class A>
+__device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set, const A& aL, const A& aR, size_t idx,
+                                          size_t nb, bool recL, bool recR, bool flux, int q, bool& any_bad) {
+    const size_t fplane = (size_t)I.mx * I.my;
+    constexpr int fL = DIR == 0 ? FACE_E : FACE_N, fR = DIR == 0 ? FACE_W : FACE_S;
+    const int sc = I.scheme[idx], sn = I.scheme[nb];
+    const int rr_now = sc < sn ? sc : sn;
+    const int rrL = recL ? rr_now : I.faceRR[fL * fplane + nb];
+    const int rrR = recR ? rr_now : I.faceRR[fR * fplane + idx];
+    const int nqL = (recL && rrL > 0) ? gp_nq(rrL) : 0;
+    const int nqR = (recR && rrR > 0) ? gp_nq(rrR) : 0;
+    const int nqF = flux ? (rr_now == 0 ? 1 : gp_nq(rr_now)) : 0;
+    double uL[NCONS], uR[NCONS], t[NCONS];
+    int badL = 0, badR = 0;
+    if (q < nqL || q < nqF) side_value<R>(aL, set, rrL, fL, q, uL);
+    if (q < nqR || q < nqF) side_value<R>(aR, set, rrR, fR, q, uR);
+    if (q < nqL) badL = !admissible(uL[DENS], cons2pres(ph, uL));
+    if (q < nqR) badR = !admissible(uR[DENS], cons2pres(ph, uR));
+    if (q < nqF) {
+        double F[NCONS];
+        hllc<DIR>(ph, uL, uR, F);
+        const double w = c_qw[set][rr_now][q];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) t[v] = rr_now == 0 ? F[v] : w * F[v];
+    } else {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) t[v] = 0.0;
+    }
+    // every lane of the quad takes part in the exchanges
+    const int base = (int)(threadIdx.x & 63u) & ~3;
+    double acc[NCONS];
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) {
+        const double t1 = __shfl(t[v], base + 1), t2 = __shfl(t[v], base + 2), t3 = __shfl(t[v], base + 3);
+        double a = rr_now == 0 ? t[v] : 0.0 + t[v];
+        if (nqF > 1) a = a + t1;
+        if (nqF > 2) a = a + t2;
+        if (nqF > 3) a = a + t3;
+        acc[v] = a;
+    }
+    badL |= __shfl_xor(badL, 1);
+    badL |= __shfl_xor(badL, 2);
+    badR |= __shfl_xor(badR, 1);
+    badR |= __shfl_xor(badR, 2);
+    if (q != 0) return;
+    if (recL) {
+        I.faceRR[fL * fplane + nb] = (uint8_t)rr_now;
+        I.inadm[fL * fplane + nb] = (uint8_t)badL;
+    }
+    if (recR) {
+        I.faceRR[fR * fplane + idx] = (uint8_t)rr_now;
+        I.inadm[fR * fplane + idx] = (uint8_t)badR;
+    }
+    any_bad |= (recL && badL) || (recR && badR);
+    if (flux) {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) I.FR[(DIR * NCONS + v) * fplane + idx] = acc[v];
+        I.FLB[DIR * fplane + idx] = acc[DIR == 0 ? MAGY : MAGX];
+        I.fluxRR[DIR * fplane + idx] = (uint8_t)rr_now;
+    }
+}
+
 // A later MOOD iteration on one tile.  Every face the reference's pass
 // would touch -- a side to re-reconstruct and check, a flux to recompute
 // -- is classified first.  A face whose sides would be rebuilt at the radii
@@ -414,7 +455,7 @@ __device__ __forceinline__ void face_tile_iter(const Inst* __restrict__ insts, c
         }
     }
     __syncthreads();
-    const int kx = lcnt[0], ky = lcnt[1], xw = (kx + 63) & ~63;
+    const int kx = lcnt[0], ky = lcnt[1], xw = (4 * kx + 63) & ~63;
     if (kx + ky > 0) {
         // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
         const double* __restrict__ U = I.U[stage_bufs(I.rot, stage).in];
@@ -431,61 +472,24 @@ __device__ __forceinline__ void face_tile_iter(const Inst* __restrict__ insts, c
     __syncthreads();
     const Phys ph = phys_of(I);
     const int set = I.gpset;
-    // x-faces occupy slots [0, kx), y-faces [xw, xw + ky): each wave does
-    // faces of one direction
-    for (int slot = threadIdx.x; slot < xw + ky; slot += NT) {
-        const bool isx = slot < kx;
+    // Four lanes per face, lane q for Gauss point q: x-faces occupy slots
+    // [0, 4 kx), y-faces [xw, xw + 4 ky) with xw a wave boundary, so each
+    // wave does faces of one direction and a sparse pass costs one point's
+    // reconstruction and HLLC per lane
+    for (int slot = threadIdx.x; slot < xw + 4 * ky; slot += NT) {
+        const bool isx = slot < 4 * kx;
         if (!isx && slot < xw) continue;
-        const int e = isx ? lst[slot] : lst[NC + slot - xw];
+        const int q = slot & 3;
+        const int e = isx ? lst[slot >> 2] : lst[NC + ((slot - xw) >> 2)];
         const int c = e & 0xff, lx = c % TX, ly = c / TX;
         const bool recL = e & 0x100, recR = e & 0x200, fl = e & 0x400;
         const size_t idx = (size_t)(j0 + ly) * mx + (i0 + lx);
         const int cc = (ly + H) * LX + (lx + H);
         const LdsAcc<LX, LP> aR{sU + cc};
-        bool badL, badR;
-        double acc[NCONS];
-        if (isx) {
-            const size_t nb = idx - 1;
-            const int sc = I.scheme[idx], sn = I.scheme[nb];
-            const int rr_now = sc < sn ? sc : sn;
-            face_iter<R, 0>(ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, FACE_E, FACE_W, recL, recR, rr_now,
-                            I.faceRR[FACE_E * fplane + nb], I.faceRR[FACE_W * fplane + idx], fl, acc, badL, badR);
-            if (recL) {
-                I.faceRR[FACE_E * fplane + nb] = (uint8_t)rr_now;
-                I.inadm[FACE_E * fplane + nb] = badL;
-            }
-            if (recR) {
-                I.faceRR[FACE_W * fplane + idx] = (uint8_t)rr_now;
-                I.inadm[FACE_W * fplane + idx] = badR;
-            }
-            if (fl) {
-#pragma unroll
-                for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
-                I.FLB[idx] = acc[MAGY];
-                I.fluxRR[idx] = (uint8_t)rr_now;
-            }
-        } else {
-            const size_t nb = idx - mx;
-            const int sc = I.scheme[idx], sn = I.scheme[nb];
-            const int rr_now = sc < sn ? sc : sn;
-            face_iter<R, 1>(ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, FACE_N, FACE_S, recL, recR, rr_now,
-                            I.faceRR[FACE_N * fplane + nb], I.faceRR[FACE_S * fplane + idx], fl, acc, badL, badR);
-            if (recL) {
-                I.faceRR[FACE_N * fplane + nb] = (uint8_t)rr_now;
-                I.inadm[FACE_N * fplane + nb] = badL;
-            }
-            if (recR) {
-                I.faceRR[FACE_S * fplane + idx] = (uint8_t)rr_now;
-                I.inadm[FACE_S * fplane + idx] = badR;
-            }
-            if (fl) {
-#pragma unroll
-                for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
-                I.FLB[fplane + idx] = acc[MAGX];
-                I.fluxRR[fplane + idx] = (uint8_t)rr_now;
-            }
-        }
-        any_bad |= (recL && badL) || (recR && badR);
+        if (isx)
+            face_quad<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, q, any_bad);
+        else
+            face_quad<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, q, any_bad);
     }
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
