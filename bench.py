@@ -403,7 +403,7 @@ def main() -> int:
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--served-requests", type=int, default=48)
-    ap.add_argument("--served-rate", type=float, default=6.0, help="mean arrivals per second")
+    ap.add_argument("--served-rate", type=float, default=12.0, help="mean arrivals per second")
     ap.add_argument("--served-budget-gib", type=float, default=8.0)
     ap.add_argument("--served-plain", type=int, default=1, help="also run each request on its own handle")
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
