@@ -82,13 +82,15 @@ struct Item {
 constexpr int kCellT = 16;
 constexpr int kFaceTX = kCellT, kFaceTY = kCellT;
 
-// a list of tiles: `n` items, or -- when `cnt` is set -- as many as the
-// device-side count says (the active list the previous MOOD iteration's
-// scan wrote, unknown to the host when it enqueues the next iteration)
+// A list of tiles whose length is a device count: the active list the
+// previous MOOD iteration's scan wrote (its length unknown to the host
+// when it enqueues the next iteration), or a fixed list whose length is
+// also kept on the device.  swz: the launch has exactly one block per
+// entry, and blocks take the entries in xcd_tile order.
 struct TileList {
     const Item* items;
-    int n;
     const int32_t* cnt;
+    int swz;
 };
 
 struct LaunchSet {
@@ -102,8 +104,14 @@ struct LaunchSet {
     int ncell;
     Item* act[2];        // active tiles, written by alternate MOOD iterations
     int32_t* act_count;  // [2] their counts
+    const int32_t* npad_dev;  // npad, on the device (the length of `pad` as a TileList)
     uint32_t* marks;     // per tile: stamp of the last iteration that may change its flags
     unsigned long long* cflpart;  // two partial maxima per CFL block
+    // One solve in the table: the per-iteration kernels take its record
+    // from their arguments (`inst1`, as the device table holds it when the
+    // kernel runs) instead of loading it after the tile.
+    int one;
+    Inst inst1;
 };
 
 }  // namespace hmhd

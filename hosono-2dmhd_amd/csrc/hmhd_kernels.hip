@@ -132,11 +132,28 @@ enum : int { FACE_N = 0, FACE_S = 1, FACE_E = 2, FACE_W = 3 };
 // the point loop (2 x 13 x 8 doubles at R = 2 would not fit).
 __device__ __forceinline__ void reload_barrier() { asm volatile("" ::: "memory"); }
 
+// The control words of one solve, read in one 16-byte load.  Within a
+// kernel the words a kernel reads do not change (a kernel writes only
+// words it does not read).
+struct Ctl {
+    int32_t any_inadm, any_lower, run, mode;
+};
+static_assert(CTL_ANY_INADM == 0 && CTL_ANY_LOWER == 1 && CTL_RUN == 2 && CTL_MODE == 3, "Ctl field order");
+__device__ __forceinline__ Ctl ctl_of(const int32_t* c) {
+    const int4 v = *reinterpret_cast<const int4*>(c);
+    return Ctl{v.x, v.y, v.z, v.w};
+}
+
 // the rest of a MOOD pass (fluxes, update, detect, lowering, sync) runs
 // only for instances inside their MOOD loop whose inner admissibility loop
 // has converged
-__device__ __forceinline__ bool pass_ready(const Inst& I) {
-    return I.ctl[CTL_RUN] != 0 && I.ctl[CTL_ANY_INADM] == 0;
+__device__ __forceinline__ bool pass_ready(const Ctl& c) { return c.run != 0 && c.any_inadm == 0; }
+
+// The record of a tile's solve: with ONE (a single solve) the kernel
+// argument, so its fields do not wait for the tile's list entry.
+template <bool ONE>
+__device__ __forceinline__ const Inst& inst_at(const Inst* __restrict__ insts, const Inst& one, int k) {
+    return ONE ? one : insts[k];
 }
 
 // Workgroups go to the 8 XCDs round-robin, and each XCD has its own L2.
@@ -150,8 +167,28 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return x * q + (x < r ? x : r) + b / kXcd;
 }
 
-// tile of loop step b: swizzled when the grid covers the list once
-__device__ __forceinline__ int list_tile(int b, int n) { return (int)gridDim.x >= n ? xcd_tile(b, n) : b; }
+// Marks values as used at this point, so their loads are issued here
+// (together with whatever else is in flight) rather than sunk past a branch.
+__device__ __forceinline__ void use_now(int a) { asm volatile("" ::"v"(a)); }
+
+// The tiles of list `tl` this block takes: entries b, b + G, ... below the
+// count.  The block's first entry, the count and the caller's `pin` value
+// (its control words) are loaded together, in one round trip; the launch
+// has at most as many blocks as the list has room for.
+template <class F>
+__device__ __forceinline__ void for_tiles(const TileList& tl, F&& body, int pin = 0) {
+    const int b0 = blockIdx.x, G = gridDim.x;
+    Item it = tl.items[tl.swz ? xcd_tile(b0, G) : b0];
+    const int n = *tl.cnt;
+    use_now(it.inst);
+    use_now(it.tx);
+    use_now(pin);
+    for (int b = b0; b < n;) {
+        body(it);
+        b += G;
+        if (b < n) it = tl.items[b];  // only lists without swz have more entries than blocks
+    }
+}
 
 // SSP-RK3 buffer roles from the solve's rotation a = I.rot (U^n in U[a]):
 // stage 0 reads U[a] and writes U1 to U[a+1]; stage 1 reads U[a+1] and
@@ -234,12 +271,10 @@ constexpr int face_lds_cells() {
 }
 
 template <int R>
-__device__ __forceinline__ void face_tile_full(const Inst* __restrict__ insts, const Item it, int stage,
-                                          double* __restrict__ sU) {
+__device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int stage, double* __restrict__ sU) {
     constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, true);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
-    const Inst& I = insts[it.inst];
     if (I.R != R) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
@@ -413,18 +448,17 @@ __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set
 // remaining faces are compacted, x-faces then y-faces from a wave boundary,
 // so lanes do not idle behind untouched faces.
 template <int R>
-__device__ __forceinline__ void face_tile_iter(const Inst* __restrict__ insts, const Item it, int stage,
+__device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int stage,
                                                double* __restrict__ sU, uint16_t* __restrict__ lst,
                                                int* __restrict__ lcnt) {
     constexpr int TX = kFaceTX, TY = kFaceTY, NC = TX * TY, H = R + 1, NT = face_threads(R, false);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
-    const Inst& I = insts[it.inst];
-    if (I.R != R || !I.ctl[CTL_RUN]) return;
+    if (I.R != R || !cw.run) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
     const int i0 = it.tx * TX, j0 = it.ty * TY;
     const size_t fplane = (size_t)mx * I.my;
-    const uint8_t* flag = I.ctl[CTL_MODE] == MODE_NEWPASS ? I.rc : I.rec;
+    const uint8_t* flag = cw.mode == MODE_NEWPASS ? I.rc : I.rec;
     if (threadIdx.x < 2) lcnt[threadIdx.x] = 0;
     __syncthreads();
 
@@ -494,57 +528,68 @@ __device__ __forceinline__ void face_tile_iter(const Inst* __restrict__ insts, c
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
 
-template <int R, bool FULL>
+template <int R, bool FULL, bool ONE>
 __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
-    const Inst* __restrict__ insts, const TileList tl, int stage) {
+    const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
     __shared__ uint16_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
     __shared__ int lcnt[2];
-    const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) {
-        if (b != (int)blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
-        const Item it = tl.items[list_tile(b, n)];
+    const Ctl c1 = ONE && !FULL ? ctl_of(one.ctl) : Ctl{};
+    bool later = false;
+    for_tiles(tl, [&](const Item it) {
+        if (later) __syncthreads();  // the previous tile's LDS reads are done
+        later = true;
+        const Inst& I = inst_at<ONE>(insts, one, it.inst);
         if (FULL)
-            face_tile_full<R>(insts, it, stage, sU);
+            face_tile_full<R>(I, it, stage, sU);
         else
-            face_tile_iter<R>(insts, it, stage, sU, lst, lcnt);
-    }
+            face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
+    }, c1.run | c1.mode);
 }
 
-__device__ __forceinline__ uint8_t gather_or(const uint8_t* a, int i, int j, int mx, int my, int rad) {
-    uint8_t r = 0;
-    for (int dj = -rad; dj <= rad; dj++) {
-        const int jj = j + dj;
-        if (jj < 0 || jj >= my) continue;
-        const int w = rad - (dj < 0 ? -dj : dj);
-        for (int di = -w; di <= w; di++) {
-            const int ii = i + di;
-            if (ii < 0 || ii >= mx) continue;
-            r |= a[(size_t)jj * mx + ii];
-        }
+// The 13 cells at taxicab distance <= 2 of (i, j), the cross (distance
+// <= 1) first.  Gathers read all of them unconditionally from clamped
+// indices and mask the ones off the grid, so their loads are in flight
+// together instead of one bounds-checked load at a time.
+constexpr int kDiamDx[13] = {0, -1, 1, 0, 0, -2, 2, 0, 0, -1, 1, -1, 1};
+constexpr int kDiamDy[13] = {0, 0, 0, -1, 1, 0, 0, -2, 2, -1, -1, 1, 1};
+
+struct Nbhd {
+    size_t off[13];  // clamped offsets
+    bool in[13];     // on the grid
+};
+__device__ __forceinline__ Nbhd nbhd(int i, int j, int mx, int my) {
+    Nbhd n;
+#pragma unroll
+    for (int k = 0; k < 13; k++) {
+        const int ii = i + kDiamDx[k], jj = j + kDiamDy[k];
+        n.in[k] = ii >= 0 && ii < mx && jj >= 0 && jj < my;
+        const int ci = ii < 0 ? 0 : (ii >= mx ? mx - 1 : ii), cj = jj < 0 ? 0 : (jj >= my ? my - 1 : jj);
+        n.off[k] = (size_t)cj * mx + ci;
     }
-    return r;
+    return n;
+}
+
+// OR of byte plane `a` over the cross and over the diamond of (i, j)
+__device__ __forceinline__ void gather_or(const uint8_t* __restrict__ a, const Nbhd& n, uint8_t& cross,
+                                          uint8_t& diam) {
+    uint8_t v[13];
+#pragma unroll
+    for (int k = 0; k < 13; k++) v[k] = a[n.off[k]];
+    uint8_t c = 0, d = 0;
+#pragma unroll
+    for (int k = 0; k < 13; k++) {
+        const uint8_t x = n.in[k] ? v[k] : 0;
+        if (k < 5) c |= x;
+        d |= x;
+    }
+    cross = c;
+    diam = c | d;
 }
 
 // a cell is inadmissible when any of its four face sides is
 __device__ __forceinline__ uint8_t inadm_cell(const Inst& I, size_t idx, size_t fplane) {
     return I.inadm[idx] | I.inadm[fplane + idx] | I.inadm[2 * fplane + idx] | I.inadm[3 * fplane + idx];
-}
-
-__device__ __forceinline__ uint8_t gather_inadm(const Inst& I, int i, int j, int rad) {
-    const size_t fplane = (size_t)I.mx * I.my;
-    uint8_t r = 0;
-    for (int dj = -rad; dj <= rad; dj++) {
-        const int jj = j + dj;
-        if (jj < 0 || jj >= I.my) continue;
-        const int w = rad - (dj < 0 ? -dj : dj);
-        for (int di = -w; di <= w; di++) {
-            const int ii = i + di;
-            if (ii < 0 || ii >= I.mx) continue;
-            r |= inadm_cell(I, (size_t)jj * I.mx + ii, fplane);
-        }
-    }
-    return r;
 }
 
 // lowering of every inadmissible cell, as a gather (src/reconstruct.f90:146-171
@@ -572,51 +617,66 @@ __device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, ui
         for (int v = ta; v <= tb; v++) marks[I.tile_off + u * I.ntx + v] = stamp;
 }
 
-__device__ __forceinline__ void lower_inner_tile(const Inst* __restrict__ insts, const Item it,
+__device__ __forceinline__ void lower_inner_tile(const Inst& I, const Ctl& cw, const Item it,
                                                  uint32_t* __restrict__ marks, uint32_t stamp) {
-    const Inst& I = insts[it.inst];
-    if (!I.ctl[CTL_RUN]) return;
+    if (!cw.run) return;
     mark_candidates(I, it, marks, stamp);
-    if (I.ctl[CTL_MODE] != MODE_INNER) return;
+    if (cw.mode != MODE_INNER) return;
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i >= I.mx || j >= I.my) return;
-    const size_t idx = (size_t)j * I.mx + i;
-    const uint8_t cross = gather_inadm(I, i, j, 1);
-    const uint8_t diam = gather_inadm(I, i, j, 2);
-    if (inadm_cell(I, idx, (size_t)I.mx * I.my)) {
-        const int sc = I.scheme[idx];
-        I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
+    const size_t idx = (size_t)j * I.mx + i, fp = (size_t)I.mx * I.my;
+    // every load before any store (the byte planes may alias as far as the
+    // compiler knows)
+    const Nbhd nb = nbhd(i, j, I.mx, I.my);
+    const uint8_t bad = inadm_cell(I, idx, fp), sc = I.scheme[idx], rc = I.rc[idx], rcf = I.rcF[idx];
+    uint8_t cross = 0, diam = 0;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        uint8_t c, d;
+        gather_or(I.inadm + f * fp, nb, c, d);
+        cross |= c;
+        diam |= d;
     }
-    I.rc[idx] |= cross;
-    I.rcF[idx] |= diam;
+    if (bad) I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
+    I.rc[idx] = rc | cross;
+    I.rcF[idx] = rcf | diam;
     I.rec[idx] = cross;
 }
 
-__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const TileList tl,
-                                                     uint32_t* __restrict__ marks, uint32_t stamp) {
-    const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_inner_tile(insts, tl.items[list_tile(b, n)], marks, stamp);
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const Inst one,
+                                                     const TileList tl, uint32_t* __restrict__ marks,
+                                                     uint32_t stamp) {
+    const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
+    for_tiles(tl, [&](const Item it) {
+        const Inst& I = inst_at<ONE>(insts, one, it.inst);
+        lower_inner_tile(I, ONE ? c1 : ctl_of(I.ctl), it, marks, stamp);
+   }, c1.run | c1.any_inadm | c1.mode);
 }
 
 // after detect: rc = cross(low), rcF = diamond(low), scheme lowered
 // (src/detect.f90:60-72,219-234)
-__device__ __forceinline__ void lower_detect_tile(const Inst* __restrict__ insts, const Item it) {
-    const Inst& I = insts[it.inst];
-    if (!pass_ready(I)) return;
+__device__ __forceinline__ void lower_detect_tile(const Inst& I, const Ctl& cw, const Item it) {
+    if (!pass_ready(cw)) return;
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i >= I.mx || j >= I.my) return;
     const size_t idx = (size_t)j * I.mx + i;
-    I.rc[idx] = gather_or(I.low, i, j, I.mx, I.my, 1);
-    I.rcF[idx] = gather_or(I.low, i, j, I.mx, I.my, 2);
-    if (I.low[idx]) {
-        const int sc = I.scheme[idx];
-        I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
-    }
+    const uint8_t low = I.low[idx], sc = I.scheme[idx];  // loads before stores
+    uint8_t cross, diam;
+    gather_or(I.low, nbhd(i, j, I.mx, I.my), cross, diam);
+    I.rc[idx] = cross;
+    I.rcF[idx] = diam;
+    if (low) I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
 }
 
-__global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const TileList tl) {
-    const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) lower_detect_tile(insts, tl.items[list_tile(b, n)]);
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const Inst one,
+                                                      const TileList tl) {
+    const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
+    for_tiles(tl, [&](const Item it) {
+        const Inst& I = inst_at<ONE>(insts, one, it.inst);
+        lower_detect_tile(I, ONE ? c1 : ctl_of(I.ctl), it);
+   }, c1.run | c1.any_inadm | c1.mode);
 }
 
 // Periodic MOOD sync (src/boundaryConditions.f90:205-252) in one launch.
@@ -641,9 +701,10 @@ __device__ __forceinline__ void sync_pair(const Inst& I, size_t a, size_t b) {
     I.scheme[b] = m;
 }
 
-__global__ __launch_bounds__(256) void k_sync(const Inst* __restrict__ insts) {
-    const Inst& I = insts[blockIdx.y];
-    if (I.bc != BC_PERIODIC || !I.active || !pass_ready(I)) return;
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_sync(const Inst* __restrict__ insts, const Inst one) {
+    const Inst& I = inst_at<ONE>(insts, one, blockIdx.y);
+    if (I.bc != BC_PERIODIC || !I.active || !pass_ready(ctl_of(I.ctl))) return;
     const int s = I.ngc, g2 = 2 * s, nx = I.nx, ny = I.ny;
     const size_t mx = (size_t)I.mx;
     const int nq = g2 * g2, nxo = g2 * (ny - g2), nyo = g2 * (nx - g2);
@@ -805,9 +866,8 @@ __device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const
 // around it -- before using any of it, so the global loads overlap.  The
 // arithmetic and its order are src/FLICDdivB.f90:38-54 and
 // src/timeStep.f90:107-118 as written.
-__device__ __forceinline__ void update_tile(const Inst* __restrict__ insts, const Item it, int stage) {
-    const Inst& I = insts[it.inst];
-    if (!pass_ready(I)) return;
+__device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const Item it, int stage) {
+    if (!pass_ready(cw)) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i < s || j < s || i > ex + 1 || j > ey + 1) return;
@@ -892,9 +952,14 @@ __device__ __forceinline__ void update_tile(const Inst* __restrict__ insts, cons
     }
 }
 
-__global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const TileList tl, int stage) {
-    const int n = tl.cnt ? *tl.cnt : tl.n;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) update_tile(insts, tl.items[list_tile(b, n)], stage);
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
+                                                int stage) {
+    const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
+    for_tiles(tl, [&](const Item it) {
+        const Inst& I = inst_at<ONE>(insts, one, it.inst);
+        update_tile(I, ONE ? c1 : ctl_of(I.ctl), it, stage);
+   }, c1.run | c1.any_inadm | c1.mode);
 }
 
 // Start of a stage's MOOD loop, one thread per instance (inactive
@@ -927,7 +992,7 @@ __global__ void k_begin(const Inst* __restrict__ insts, int ninst, int stage, in
 __global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, int ninst, int stage,
                                                 int32_t* __restrict__ ctl, int w, int32_t* __restrict__ snap) {
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
-        int32_t* c = insts[k].ctl;
+        int32_t* c = ctl + CTL_N * k;
         if (!c[CTL_RUN]) continue;
         if (c[CTL_ANY_INADM]) {
             c[CTL_ANY_INADM] = 0;
@@ -952,21 +1017,34 @@ __global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, 
 // within two cells of a cell flagged rc, rcF or inadm, so a tile takes part
 // when its footprint grown by two cells holds such a cell.  Appended in any
 // order: each tile's work is independent of the others'.
-__global__ __launch_bounds__(256) void k_tile_scan(const Inst* __restrict__ insts, const Item* __restrict__ items,
-                                                   Item* __restrict__ act, int32_t* __restrict__ act_count,
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
+                                                   const Item* __restrict__ items, Item* __restrict__ act,
+                                                   int32_t* __restrict__ act_count,
                                                    const uint32_t* __restrict__ marks, uint32_t stamp) {
-    const Item it = items[xcd_tile(blockIdx.x, gridDim.x)];
-    const Inst& I = insts[it.inst];
+    // one block per tile of the padded grids; a single solve's tiles are
+    // its rows of tiles in order, so the tile follows from the block
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const Item it = ONE ? Item{0, (int16_t)(b % one.ntx), (int16_t)(b / one.ntx)} : items[b];
+    const Inst& I = inst_at<ONE>(insts, one, it.inst);
     if (stamp != 0 && marks[I.tile_off + it.ty * I.ntx + it.tx] != stamp) return;
     const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
-    constexpr int W = kCellT + 4;
-    bool any = false;
-    for (int c = threadIdx.x; c < W * W; c += blockDim.x) {
-        const int i = x0 + c % W, j = y0 + c / W;
-        if (i < 0 || j < 0 || i >= I.mx || j >= I.my) continue;
-        const size_t idx = (size_t)j * I.mx + i;
-        any |= (I.rc[idx] | I.rcF[idx] | inadm_cell(I, idx, (size_t)I.mx * I.my)) != 0;
+    constexpr int W = kCellT + 4, NT = 256, NR = (W * W + NT - 1) / NT;
+    const size_t fp = (size_t)I.mx * I.my;
+    // every cell of the footprint read at once (clamped, masked)
+    uint8_t f[NR];
+    bool in[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int c = threadIdx.x + r * NT, i = x0 + c % W, j = y0 + c / W;
+        in[r] = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
+        const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
+        const size_t idx = (size_t)cj * I.mx + ci;
+        f[r] = I.rc[idx] | I.rcF[idx] | inadm_cell(I, idx, fp);
     }
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
     if (__syncthreads_or(any) && threadIdx.x == 0) act[atomicAdd(act_count, 1)] = it;
 }
 
@@ -1178,49 +1256,70 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, L.insts, L.ninst, stage, L.ctl, w, snap);
 }
 
+// the single-solve variant of a kernel when the table holds one solve
+#define HMHD_LAUNCH1(K, grid, block, ...)                                                   \
+    do {                                                                                    \
+        if (L.one)                                                                          \
+            hipLaunchKernelGGL(K<true>, grid, block, 0, st, L.insts, L.inst1, __VA_ARGS__); \
+        else                                                                                \
+            hipLaunchKernelGGL(K<false>, grid, block, 0, st, L.insts, L.inst1, __VA_ARGS__); \
+    } while (0)
+
 void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st) {
     if (L.npad <= 0) return;
-    hipLaunchKernelGGL(k_tile_scan, g1(L.npad), dim3(256), 0, st, L.insts, L.pad, L.act[w], L.act_count + w, L.marks,
-                       stamp);
+    HMHD_LAUNCH1(k_tile_scan, g1(L.npad), dim3(256), L.pad, L.act[w], L.act_count + w, L.marks, stamp);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
     if (grid <= 0) return;
     const dim3 g = g1(grid);
-#define HMHD_FACE(RR)                                                                                        \
+#define HMHD_FACE(RR, ONE)                                                                                   \
     if (full)                                                                                                \
-        hipLaunchKernelGGL((k_face<RR, true>), g, dim3(face_threads(RR, true)), 0, st, L.insts, tl, stage);    \
+        hipLaunchKernelGGL((k_face<RR, true, ONE>), g, dim3(face_threads(RR, true)), 0, st, L.insts, L.inst1, \
+                           tl, stage);                                                                       \
     else                                                                                                     \
-        hipLaunchKernelGGL((k_face<RR, false>), g, dim3(face_threads(RR, false)), 0, st, L.insts, tl, stage);
-    if (R == 1) {
-        HMHD_FACE(1)
-    } else if (R == 2) {
-        HMHD_FACE(2)
-    } else {
-        HMHD_FACE(3)
+        hipLaunchKernelGGL((k_face<RR, false, ONE>), g, dim3(face_threads(RR, false)), 0, st, L.insts,        \
+                           L.inst1, tl, stage);
+#define HMHD_FACE_R(ONE)   \
+    if (R == 1) {          \
+        HMHD_FACE(1, ONE)  \
+    } else if (R == 2) {   \
+        HMHD_FACE(2, ONE)  \
+    } else {               \
+        HMHD_FACE(3, ONE)  \
     }
+    if (L.one) {
+        HMHD_FACE_R(true)
+    } else {
+        HMHD_FACE_R(false)
+    }
+#undef HMHD_FACE_R
 #undef HMHD_FACE
 }
 
 void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st) {
     if (grid <= 0) return;
-    hipLaunchKernelGGL(k_lower_inner, g1(grid), dim3(256), 0, st, L.insts, tl, L.marks, stamp);
+    HMHD_LAUNCH1(k_lower_inner, g1(grid), dim3(256), tl, L.marks, stamp);
 }
 
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st) {
     if (grid <= 0) return;
-    hipLaunchKernelGGL(k_update, g1(grid), dim3(256), 0, st, L.insts, tl, stage);
+    HMHD_LAUNCH1(k_update, g1(grid), dim3(256), tl, stage);
 }
 
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st) {
     if (grid <= 0) return;
-    hipLaunchKernelGGL(k_lower_detect, g1(grid), dim3(256), 0, st, L.insts, tl);
+    HMHD_LAUNCH1(k_lower_detect, g1(grid), dim3(256), tl);
 }
 
 void launch_sync(const LaunchSet& L, hipStream_t st) {
     if (L.sync_gx <= 0) return;
-    hipLaunchKernelGGL(k_sync, dim3(L.sync_gx, L.ninst), dim3(256), 0, st, L.insts);
+    if (L.one)
+        hipLaunchKernelGGL(k_sync<true>, dim3(L.sync_gx, L.ninst), dim3(256), 0, st, L.insts, L.inst1);
+    else
+        hipLaunchKernelGGL(k_sync<false>, dim3(L.sync_gx, L.ninst), dim3(256), 0, st, L.insts, L.inst1);
 }
+#undef HMHD_LAUNCH1
 
 void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st) {
     if (L.ncell > 0) hipLaunchKernelGGL(k_cfl, g1(L.ncell), dim3(256), 0, st, L.insts, L.cell, L.cflpart);

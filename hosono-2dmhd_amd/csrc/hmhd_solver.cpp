@@ -374,9 +374,17 @@ int Engine::commit() {
     L.act[0] = d_act_;
     L.act[1] = d_act_ + pad.size() + 1;
     L.act_count = d_ctl_ + (size_t)CTL_N * n;
+    L.npad_dev = d_ctl_ + (size_t)CTL_N * n + 2;
+    npad_val_ = (int32_t)pad.size();
+    if ((rc = hip_check(hipMemcpyAsync(d_ctl_ + (size_t)CTL_N * n + 2, &npad_val_, sizeof npad_val_,
+                                       hipMemcpyHostToDevice, st_),
+                        "hipMemcpyAsync(npad)")))
+        return rc;
     L.marks = d_marks_;
     est_act_ = L.npad;
     L.cflpart = d_part_;
+    L.one = n == 1 ? 1 : 0;
+    L.inst1 = n == 1 ? table[0] : Inst{};
     ctl_.assign((size_t)CTL_N * n, 0);
     dirty_ = false;
     return sync();
@@ -489,7 +497,7 @@ void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
 // snapshot into host slot it & 1.  `grid` blocks stride over the list.
 int Engine::enqueue_iteration(int s, int it, int grid) {
     const int r = (it - 1) & 1, w = it & 1;
-    const TileList act{L.act[r], 0, L.act_count + r};
+    const TileList act{L.act[r], L.act_count + r, 0};
     // candidate stamps are never 0 (0 asks the scan for every tile)
     if (++stamp_ == 0) stamp_ = 1;
     prof_begin(PF_RECON);
@@ -519,7 +527,7 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
 // and the pass counts are those of the plain loop.
 int Engine::run_stage(int s) {
     const int n = (int)inst_.size();
-    const TileList all{L.pad, L.npad, nullptr};
+    const TileList all{L.pad, L.npad_dev, 1};
     // iteration 0, the first pass: every tile, every face at the top radius
     launch_begin(L, s, st_);
     launch_stage_init(L, st_);
@@ -596,10 +604,20 @@ int Engine::advance(const std::vector<double>& dt) {
                         "hipMemcpyAsync(dt)")))
         return rc;
     launch_set_dt(d_insts_, d_dt_, n, st_);
+    if (L.one) {
+        // the single-solve kernels' copy of the record, as k_set_dt writes it
+        L.inst1.dtdx = h_dt_[0] / L.inst1.dx;
+        L.inst1.dtdy = h_dt_[0] / L.inst1.dy;
+    }
     // U1 = FE(U); U2 = (3U + FE(U1))/4; U^{n+1} = (U + 2 FE(U2))/3
     for (int s = 0; s < 3; s++)
         if ((rc = run_stage(s))) return rc;
     launch_rotate(d_insts_, n, st_);
+    if (L.one && L.inst1.active) {
+        // as k_rotate
+        L.inst1.rot = (L.inst1.rot + 1) % 3;
+        L.inst1.v0 = 0;
+    }
     for (int k = 0; k < n; k++) {
         InstHost& h = inst_[k];
         if (!h.active) continue;

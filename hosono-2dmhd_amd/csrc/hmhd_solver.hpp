@@ -147,6 +147,7 @@ class Engine {
     size_t cap_stage_i_ = 0;
     LaunchSet L{};
     int act_n_ = 0;    // length of the active list in the latest snapshot
+    int32_t npad_val_ = 0;  // source of the device copy of L.npad
     int est_act_ = 0;  // active tiles after the previous stage's first pass
     bool has_R_[kMaxR + 1] = {false};
     std::vector<int32_t> ctl_;
