@@ -170,6 +170,11 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // Marks values as used at this point, so their loads are issued here
 // (together with whatever else is in flight) rather than sunk past a branch.
 __device__ __forceinline__ void use_now(int a) { asm volatile("" ::"v"(a)); }
+__device__ __forceinline__ void use_now(double a) { asm volatile("" ::"v"(a)); }
+template <class... T>
+__device__ __forceinline__ void use_all(T... v) {
+    (use_now(v), ...);
+}
 
 // The tiles of list `tl` this block takes: entries b, b + G, ... below the
 // count.  The block's first entry, the count and the caller's `pin` value
@@ -198,11 +203,13 @@ struct StageBufs {
     int in, out, base;
 };
 __device__ __forceinline__ StageBufs stage_bufs(int rot, int stage) {
-    const int a = rot, b = (rot + 1) % 3, c = (rot + 2) % 3;
-    if (stage == 0) return {a, b, a};
-    if (stage == 1) return {b, c, a};
-    return {c, b, a};
+    // in = rot + stage, out = rot + 1 (rot + 2 at stage 1), base = rot, mod 3
+    const int in = rot + stage, out = rot + 1 + (stage == 1 ? 1 : 0);
+    return {in >= 3 ? in - 3 : in, out >= 3 ? out - 3 : out, rot};
 }
+
+// state buffer k of a solve, selected without indexing the pointer array
+__device__ __forceinline__ double* ubuf(const Inst& I, int k) { return k == 0 ? I.U[0] : (k == 1 ? I.U[1] : I.U[2]); }
 
 template <int DIR>
 __device__ __forceinline__ void accumulate_flux(double* acc, const Phys& ph, const double* uL, const double* uR,
@@ -293,7 +300,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
 
     // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
     {
-        const double* __restrict__ U = I.U[stage_bufs(I.rot, stage).in];
+        const double* __restrict__ U = ubuf(I, stage_bufs(I.rot, stage).in);
         const size_t plane = (size_t)I.nx * I.ny;
         const int oi = i0 - H, oj = j0 - H;
         for (int c = threadIdx.x; c < LP; c += NT) {
@@ -375,15 +382,16 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
 // last rebuilt with), check its admissibility, and form w_q F_HLLC; lane 0
 // sums the points in ascending order from zero (or takes the one
 // first-order flux as it is) and writes the face's results.
+// rr_now: the face's radius, min of its cells' schemes; storedL/R: the
+// radius each side was last reconstructed with.
 template <int R, int DIR, class A>
 __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set, const A& aL, const A& aR, size_t idx,
-                                          size_t nb, bool recL, bool recR, bool flux, int q, bool& any_bad) {
+                                          size_t nb, bool recL, bool recR, bool flux, int rr_now, int storedL,
+                                          int storedR, int q, bool& any_bad) {
     const size_t fplane = (size_t)I.mx * I.my;
     constexpr int fL = DIR == 0 ? FACE_E : FACE_N, fR = DIR == 0 ? FACE_W : FACE_S;
-    const int sc = I.scheme[idx], sn = I.scheme[nb];
-    const int rr_now = sc < sn ? sc : sn;
-    const int rrL = recL ? rr_now : I.faceRR[fL * fplane + nb];
-    const int rrR = recR ? rr_now : I.faceRR[fR * fplane + idx];
+    const int rrL = recL ? rr_now : storedL;
+    const int rrR = recR ? rr_now : storedR;
     const int nqL = (recL && rrL > 0) ? gp_nq(rrL) : 0;
     const int nqR = (recR && rrR > 0) ? gp_nq(rrR) : 0;
     const int nqF = flux ? (rr_now == 0 ? 1 : gp_nq(rr_now)) : 0;
@@ -449,7 +457,7 @@ __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set
 // so lanes do not idle behind untouched faces.
 template <int R>
 __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int stage,
-                                               double* __restrict__ sU, uint16_t* __restrict__ lst,
+                                               double* __restrict__ sU, uint32_t* __restrict__ lst,
                                                int* __restrict__ lcnt) {
     constexpr int TX = kFaceTX, TY = kFaceTY, NC = TX * TY, H = R + 1, NT = face_threads(R, false);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
@@ -466,33 +474,45 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     for (int f = threadIdx.x; f < 2 * NC; f += NT) {
         const int d = f / NC, c = f - d * NC;
         const int i = i0 + c % TX, j = j0 + c / TX;
-        if (i < s - 2 || i > ex + 3 || j < s - 2 || j > ey + 3) continue;
-        const size_t idx = (size_t)j * mx + i, nb = d == 0 ? idx - 1 : idx - mx;
+        const bool dom = i >= s - 2 && i <= ex + 3 && j >= s - 2 && j <= ey + 3;
+        // every value the classification reads, loaded at once from clamped
+        // indices (a cell off the range reads a valid cell and is dropped)
+        const int ci = i < 1 ? 1 : (i >= mx ? mx - 1 : i), cj = j < 1 ? 1 : (j >= I.my ? I.my - 1 : j);
+        const size_t idx = (size_t)cj * mx + ci, nb = d == 0 ? idx - 1 : idx - mx;
+        const int fL = d == 0 ? FACE_E : FACE_N, fR = d == 0 ? FACE_W : FACE_S;
+        const uint8_t flagR = flag[idx], flagL = flag[nb], rcR = I.rc[idx], rcL = I.rc[nb];
+        const uint8_t sc = I.scheme[idx], sn = I.scheme[nb];
+        const uint8_t rrL = I.faceRR[fL * fplane + nb], rrR = I.faceRR[fR * fplane + idx];
+        const uint8_t rrF = I.fluxRR[d * fplane + idx];
+        const uint8_t badL = I.inadm[fL * fplane + nb], badR = I.inadm[fR * fplane + idx];
+        const double raw = I.FR[(d * NCONS + (d == 0 ? MAGY : MAGX)) * fplane + idx];
+        use_all((int)flagR, (int)flagL, (int)rcR, (int)rcL, (int)sc, (int)sn, (int)rrL, (int)rrR, (int)rrF, (int)badL,
+                (int)badR, raw);
+        if (!dom) continue;
         // the face's cells within the reconstruction range [s-2, e+2]^2
         const bool cellR = i <= ex + 2 && j <= ey + 2;
         const bool cellL = d == 0 ? (i >= s - 1 && j <= ey + 2) : (j >= s - 1 && i <= ex + 2);
-        const bool recR = cellR && flag[idx], recL = cellL && flag[nb];
-        const bool fl = i >= s - 1 && i <= ex + 2 && j >= s - 1 && j <= ey + 2 && (I.rc[idx] || I.rc[nb]);
+        const bool recR = cellR && flagR, recL = cellL && flagL;
+        const bool fl = i >= s - 1 && i <= ex + 2 && j >= s - 1 && j <= ey + 2 && (rcR || rcL);
         if (!(recL || recR || fl)) continue;
-        const int fL = d == 0 ? FACE_E : FACE_N, fR = d == 0 ? FACE_W : FACE_S;
-        const int sc = I.scheme[idx], sn = I.scheme[nb];
         const int rr_now = sc < sn ? sc : sn;
-        const bool keepL = !recL || I.faceRR[fL * fplane + nb] == rr_now;
-        const bool keepR = !recR || I.faceRR[fR * fplane + idx] == rr_now;
-        const bool keepF = !fl || I.fluxRR[d * fplane + idx] == rr_now;
+        const bool keepL = !recL || rrL == rr_now;
+        const bool keepR = !recR || rrR == rr_now;
+        const bool keepF = !fl || rrF == rr_now;
         if (keepL && keepR && keepF) {
-            if (fl) I.FLB[d * fplane + idx] = I.FR[(d * NCONS + (d == 0 ? MAGY : MAGX)) * fplane + idx];
-            any_bad |= (recL && I.inadm[fL * fplane + nb]) || (recR && I.inadm[fR * fplane + idx]);
+            if (fl) I.FLB[d * fplane + idx] = raw;
+            any_bad |= (recL && badL) || (recR && badR);
         } else {
             const int slot = atomicAdd(&lcnt[d], 1);
-            lst[d * NC + slot] = (uint16_t)(c | (recL ? 0x100 : 0) | (recR ? 0x200 : 0) | (fl ? 0x400 : 0));
+            lst[d * NC + slot] = (uint32_t)c | (recL ? 0x100u : 0u) | (recR ? 0x200u : 0u) | (fl ? 0x400u : 0u) |
+                                 (uint32_t)rr_now << 12 | (uint32_t)rrL << 16 | (uint32_t)rrR << 20;
         }
     }
     __syncthreads();
     const int kx = lcnt[0], ky = lcnt[1], xw = (4 * kx + 63) & ~63;
     if (kx + ky > 0) {
         // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
-        const double* __restrict__ U = I.U[stage_bufs(I.rot, stage).in];
+        const double* __restrict__ U = ubuf(I, stage_bufs(I.rot, stage).in);
         const size_t plane = (size_t)I.nx * I.ny;
         const int oi = i0 - H, oj = j0 - H;
         for (int c = threadIdx.x; c < LP; c += NT) {
@@ -514,16 +534,19 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const bool isx = slot < 4 * kx;
         if (!isx && slot < xw) continue;
         const int q = slot & 3;
-        const int e = isx ? lst[slot >> 2] : lst[NC + ((slot - xw) >> 2)];
+        const uint32_t e = isx ? lst[slot >> 2] : lst[NC + ((slot - xw) >> 2)];
         const int c = e & 0xff, lx = c % TX, ly = c / TX;
         const bool recL = e & 0x100, recR = e & 0x200, fl = e & 0x400;
+        const int rr_now = (e >> 12) & 0xf, rrL = (e >> 16) & 0xf, rrR = (e >> 20) & 0xf;
         const size_t idx = (size_t)(j0 + ly) * mx + (i0 + lx);
         const int cc = (ly + H) * LX + (lx + H);
         const LdsAcc<LX, LP> aR{sU + cc};
         if (isx)
-            face_quad<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, q, any_bad);
+            face_quad<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
+                            q, any_bad);
         else
-            face_quad<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, q, any_bad);
+            face_quad<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
+                            rrR, q, any_bad);
     }
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
@@ -532,7 +555,7 @@ template <int R, bool FULL, bool ONE>
 __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
     const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
-    __shared__ uint16_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
+    __shared__ uint32_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
     __shared__ int lcnt[2];
     const Ctl c1 = ONE && !FULL ? ctl_of(one.ctl) : Ctl{};
     bool later = false;
@@ -688,17 +711,31 @@ __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ i
 // guard column and a guard row with their images, whose two x pairs come
 // before its two y pairs as in the sweeps.  Needs N >= 2 NGC, so the guard
 // and image bands are disjoint.  One grid row of blocks per instance.
-__device__ __forceinline__ void sync_pair(const Inst& I, size_t a, size_t b) {
-    uint8_t e = I.rcF[a] | I.rcF[b];
-    I.rcF[a] = e;
-    I.rcF[b] = e;
-    e = I.rc[a] | I.rc[b];
-    I.rc[a] = e;
-    I.rc[b] = e;
-    const uint8_t sa = I.scheme[a], sb = I.scheme[b];
-    const uint8_t m = sa < sb ? sa : sb;
-    I.scheme[a] = m;
-    I.scheme[b] = m;
+// A unit of K cells ends with rc and rcF the OR over the unit and scheme
+// its MIN (for a quad, the x pairs then the y pairs give exactly that).
+// Every value is loaded before any is stored.
+template <int K>
+__device__ __forceinline__ void sync_unit(const Inst& I, const size_t (&c)[K]) {
+    uint8_t rc[K], rcf[K], sc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        rc[k] = I.rc[c[k]];
+        rcf[k] = I.rcF[c[k]];
+        sc[k] = I.scheme[c[k]];
+    }
+    uint8_t orc = rc[0], orf = rcf[0], m = sc[0];
+#pragma unroll
+    for (int k = 1; k < K; k++) {
+        orc |= rc[k];
+        orf |= rcf[k];
+        m = sc[k] < m ? sc[k] : m;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        I.rc[c[k]] = orc;
+        I.rcF[c[k]] = orf;
+        I.scheme[c[k]] = m;
+    }
 }
 
 template <bool ONE>
@@ -722,20 +759,20 @@ __global__ __launch_bounds__(256) void k_sync(const Inst* __restrict__ insts, co
         if (u < nq) {
             col(u % g2, i, ip);
             row(u / g2, j, jp);
-            sync_pair(I, j * mx + i, j * mx + ip);
-            sync_pair(I, jp * mx + i, jp * mx + ip);
-            sync_pair(I, j * mx + i, jp * mx + i);
-            sync_pair(I, j * mx + ip, jp * mx + ip);
+            const size_t c[4] = {j * mx + i, j * mx + ip, jp * mx + i, jp * mx + ip};
+            sync_unit<4>(I, c);
         } else if (u < nq + nxo) {
             const int v = u - nq;
             col(v % g2, i, ip);
             const size_t r = (size_t)(g2 + v / g2);
-            sync_pair(I, r * mx + i, r * mx + ip);
+            const size_t c[2] = {r * mx + i, r * mx + ip};
+            sync_unit<2>(I, c);
         } else {
             const int v = u - nq - nxo;
             row(v % g2, j, jp);
-            const size_t c = (size_t)(g2 + v / g2);
-            sync_pair(I, j * mx + c, jp * mx + c);
+            const size_t cc = (size_t)(g2 + v / g2);
+            const size_t c[2] = {j * mx + cc, jp * mx + cc};
+            sync_unit<2>(I, c);
         }
     }
 }
@@ -887,30 +924,30 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     const uint8_t rcfE = I.rcF[idx + 1], rcfN = I.rcF[idx + mx];
     const double flbE = I.FLB[idx + 1], flbN = I.FLB[fp + idx + mx];
     const StageBufs sb = stage_bufs(I.rot, stage);
-    const double* __restrict__ Uin = I.U[sb.in];
+    const double* __restrict__ Uin = ubuf(I, sb.in);
     const size_t plane = (size_t)I.nx * I.ny;
+    // the cells of the last row / column (not inner) read valid addresses
+    // (fluxes of their own faces, cell 0 of U) and drop the values, so the
+    // loads need no branch and are in flight together
     const size_t ci = inner ? (size_t)(j - s) * I.nx + (i - s) : 0;
     double fxm[NCONS], fxp[NCONS], fym[NCONS], fyp[NCONS], uc[NCONS], ub[NCONS];
     Around a;
-    const int sch = inner ? I.scheme[idx] : 0;
-    if (inner) {
+    const int sch = I.scheme[idx];
 #pragma unroll
-        for (int v = 0; v < NCONS; v++) {
-            if (v == MAGX || v == MAGY) continue;
-            fxm[v] = I.FR[v * fp + idx];
-            fxp[v] = I.FR[v * fp + idx + 1];
-            fym[v] = I.FR[(NCONS + v) * fp + idx];
-            fyp[v] = I.FR[(NCONS + v) * fp + idx + mx];
-        }
-#pragma unroll
-        for (int v = 0; v < NCONS; v++) uc[v] = Uin[v * plane + ci];
-        if (stage > 0) {
-            const double* __restrict__ Ub = I.U[sb.base];
-#pragma unroll
-            for (int v = 0; v < NCONS; v++) ub[v] = Ub[v * plane + ci];
-        }
-        load_around(I, Uin, i, j, a);
+    for (int v = 0; v < NCONS; v++) {
+        if (v == MAGX || v == MAGY) continue;
+        fxm[v] = I.FR[v * fp + idx];
+        fxp[v] = I.FR[v * fp + idx + 1];
+        fym[v] = I.FR[(NCONS + v) * fp + idx];
+        fyp[v] = I.FR[(NCONS + v) * fp + idx + mx];
     }
+    const double* __restrict__ Ub = ubuf(I, sb.base);  // read at stage 0 too, unused there
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) {
+        uc[v] = Uin[v * plane + ci];
+        ub[v] = Ub[v * plane + ci];
+    }
+    load_around(I, Uin, i, j, a);
     // src/FLICDdivB.f90:38-45 and :47-54 at (i, j)
     const double cx = (2.0 * x0 + x1 + xw - y1 - y0 - yw0 - yw1) / 8.0;
     const double cy = (2.0 * y0 + y1 + ys - x1 - x0 - xs0 - xs1) / 8.0;
@@ -939,7 +976,7 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     }
     I.low[idx] = lower ? 1 : 0;
     if (lower) I.ctl[CTL_ANY_LOWER] = 1;
-    double* __restrict__ Uout = I.U[sb.out];
+    double* __restrict__ Uout = ubuf(I, sb.out);
     if (stage == 0) {
 #pragma unroll
         for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = un[v];
@@ -1018,18 +1055,19 @@ __global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, 
 // when its footprint grown by two cells holds such a cell.  Appended in any
 // order: each tile's work is independent of the others'.
 template <bool ONE>
-__global__ __launch_bounds__(256) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
-                                                   const Item* __restrict__ items, Item* __restrict__ act,
-                                                   int32_t* __restrict__ act_count,
-                                                   const uint32_t* __restrict__ marks, uint32_t stamp) {
-    // one block per tile of the padded grids; a single solve's tiles are
-    // its rows of tiles in order, so the tile follows from the block
+__global__ __launch_bounds__(64) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
+                                                  const Item* __restrict__ items, Item* __restrict__ act,
+                                                  int32_t* __restrict__ act_count,
+                                                  const uint32_t* __restrict__ marks, uint32_t stamp) {
+    // one wave per tile of the padded grids (all resident at once); a single
+    // solve's tiles are its rows of tiles in order, so the tile follows
+    // from the block
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const Item it = ONE ? Item{0, (int16_t)(b % one.ntx), (int16_t)(b / one.ntx)} : items[b];
     const Inst& I = inst_at<ONE>(insts, one, it.inst);
     if (stamp != 0 && marks[I.tile_off + it.ty * I.ntx + it.tx] != stamp) return;
     const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
-    constexpr int W = kCellT + 4, NT = 256, NR = (W * W + NT - 1) / NT;
+    constexpr int W = kCellT + 4, NT = 64, NR = (W * W + NT - 1) / NT;
     const size_t fp = (size_t)I.mx * I.my;
     // every cell of the footprint read at once (clamped, masked)
     uint8_t f[NR];
@@ -1045,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_tile_scan(const Inst* __restrict__ inst
     bool any = false;
 #pragma unroll
     for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
-    if (__syncthreads_or(any) && threadIdx.x == 0) act[atomicAdd(act_count, 1)] = it;
+    if (__ballot(any) != 0 && threadIdx.x == 0) act[atomicAdd(act_count, 1)] = it;
 }
 
 // ------------------------------------------------------------------ //
@@ -1267,7 +1305,7 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
 
 void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st) {
     if (L.npad <= 0) return;
-    HMHD_LAUNCH1(k_tile_scan, g1(L.npad), dim3(256), L.pad, L.act[w], L.act_count + w, L.marks, stamp);
+    HMHD_LAUNCH1(k_tile_scan, g1(L.npad), dim3(64), L.pad, L.act[w], L.act_count + w, L.marks, stamp);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
