@@ -1054,36 +1054,57 @@ __global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, 
 // within two cells of a cell flagged rc, rcF or inadm, so a tile takes part
 // when its footprint grown by two cells holds such a cell.  Appended in any
 // order: each tile's work is independent of the others'.
+// kScanTiles tiles per block, one wave each; the block appends its tiles
+// with one atomic, so the list count sees one update per block
+constexpr int kScanTiles = 4;
 template <bool ONE>
-__global__ __launch_bounds__(64) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
-                                                  const Item* __restrict__ items, Item* __restrict__ act,
-                                                  int32_t* __restrict__ act_count,
-                                                  const uint32_t* __restrict__ marks, uint32_t stamp) {
-    // one wave per tile of the padded grids (all resident at once); a single
-    // solve's tiles are its rows of tiles in order, so the tile follows
-    // from the block
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
-    const Item it = ONE ? Item{0, (int16_t)(b % one.ntx), (int16_t)(b / one.ntx)} : items[b];
+__global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
+                                                               const Item* __restrict__ items, int npad,
+                                                               Item* __restrict__ act, int32_t* __restrict__ act_count,
+                                                               const uint32_t* __restrict__ marks, uint32_t stamp) {
+    __shared__ int hit[kScanTiles];
+    __shared__ int base;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    // a single solve's tiles are its rows of tiles in order, so the tile
+    // follows from its index
+    const int t = xcd_tile(blockIdx.x, gridDim.x) * kScanTiles + w;
+    const int tc = t < npad ? t : npad - 1;
+    const Item it = ONE ? Item{0, (int16_t)(tc % one.ntx), (int16_t)(tc / one.ntx)} : items[tc];
     const Inst& I = inst_at<ONE>(insts, one, it.inst);
-    if (stamp != 0 && marks[I.tile_off + it.ty * I.ntx + it.tx] != stamp) return;
-    const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
-    constexpr int W = kCellT + 4, NT = 64, NR = (W * W + NT - 1) / NT;
-    const size_t fp = (size_t)I.mx * I.my;
-    // every cell of the footprint read at once (clamped, masked)
-    uint8_t f[NR];
-    bool in[NR];
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int c = threadIdx.x + r * NT, i = x0 + c % W, j = y0 + c / W;
-        in[r] = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
-        const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
-        const size_t idx = (size_t)cj * I.mx + ci;
-        f[r] = I.rc[idx] | I.rcF[idx] | inadm_cell(I, idx, fp);
-    }
     bool any = false;
+    if (t < npad && (stamp == 0 || marks[I.tile_off + it.ty * I.ntx + it.tx] == stamp)) {
+        const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
+        constexpr int W = kCellT + 4, NR = (W * W + 63) / 64;
+        const size_t fp = (size_t)I.mx * I.my;
+        // every cell of the footprint read at once (clamped, masked)
+        uint8_t f[NR];
+        bool in[NR];
 #pragma unroll
-    for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
-    if (__ballot(any) != 0 && threadIdx.x == 0) act[atomicAdd(act_count, 1)] = it;
+        for (int r = 0; r < NR; r++) {
+            const int c = lane + r * 64, i = x0 + c % W, j = y0 + c / W;
+            in[r] = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
+            const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
+            const size_t idx = (size_t)cj * I.mx + ci;
+            f[r] = I.rc[idx] | I.rcF[idx] | inadm_cell(I, idx, fp);
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
+    }
+    const bool hw = __ballot(any) != 0;
+    if (lane == 0) hit[w] = hw ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < kScanTiles; k++) n += hit[k];
+        base = n ? atomicAdd(act_count, n) : 0;
+    }
+    __syncthreads();
+    if (lane == 0 && hw) {
+        int r = 0;
+        for (int k = 0; k < w; k++) r += hit[k];
+        act[base + r] = it;
+    }
 }
 
 // ------------------------------------------------------------------ //
@@ -1305,7 +1326,8 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
 
 void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st) {
     if (L.npad <= 0) return;
-    HMHD_LAUNCH1(k_tile_scan, g1(L.npad), dim3(64), L.pad, L.act[w], L.act_count + w, L.marks, stamp);
+    HMHD_LAUNCH1(k_tile_scan, g1((L.npad + kScanTiles - 1) / kScanTiles), dim3(64 * kScanTiles), L.pad, L.npad,
+                 L.act[w], L.act_count + w, L.marks, stamp);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
