@@ -1026,8 +1026,7 @@ __global__ void k_begin(const Inst* __restrict__ insts, int ninst, int stage, in
 // src/reconstruct.f90:146-171); the active list the next iteration's scan
 // writes is emptied; then every control word and both list counts are
 // copied to host memory, where the host tests RUN.
-__global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, int ninst, int stage,
-                                                int32_t* __restrict__ ctl, int w, int32_t* __restrict__ snap) {
+__device__ __forceinline__ void loop_decisions(int32_t* __restrict__ ctl, int ninst, int stage) {
     for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
         int32_t* c = ctl + CTL_N * k;
         if (!c[CTL_RUN]) continue;
@@ -1044,6 +1043,11 @@ __global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, 
             c[CTL_RUN] = 0;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_finish(int ninst, int stage, int32_t* __restrict__ ctl, int w,
+                                                int32_t* __restrict__ snap) {
+    loop_decisions(ctl, ninst, stage);
     if (threadIdx.x == 0) ctl[CTL_N * ninst + (1 - w)] = 0;
     __syncthreads();
     for (int t = threadIdx.x; t < CTL_N * ninst + 2; t += blockDim.x) snap[t] = ctl[t];
@@ -1057,18 +1061,27 @@ __global__ __launch_bounds__(256) void k_finish(const Inst* __restrict__ insts, 
 // kScanTiles tiles per block, one wave each; the block appends its tiles
 // with one atomic, so the list count sees one update per block
 constexpr int kScanTiles = 4;
+//
+// With `snap` set, block 0 also ends the MOOD iteration as k_finish does
+// (the decisions read only flags earlier kernels set, and no scan block
+// reads a control word): the decisions, the control words copied to
+// `snap`, and, in snap's count slot w, the length of the list this
+// iteration consumed (list 1 - w, complete since the previous scan), whose
+// count it then clears for the next scan.
 template <bool ONE>
 __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
                                                                const Item* __restrict__ items, int npad,
                                                                Item* __restrict__ act, int32_t* __restrict__ act_count,
-                                                               const uint32_t* __restrict__ marks, uint32_t stamp) {
+                                                               const uint32_t* __restrict__ marks, uint32_t stamp,
+                                                               int ninst, int stage, int32_t* __restrict__ ctl,
+                                                               int wlist, int32_t* __restrict__ snap) {
     __shared__ int hit[kScanTiles];
     __shared__ int base;
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     // a single solve's tiles are its rows of tiles in order, so the tile
     // follows from its index
     const int t = xcd_tile(blockIdx.x, gridDim.x) * kScanTiles + w;
-    const int tc = t < npad ? t : npad - 1;
+    const int tc = t < npad ? t : (npad > 0 ? npad - 1 : 0);  // the list has room for one entry at least
     const Item it = ONE ? Item{0, (int16_t)(tc % one.ntx), (int16_t)(tc / one.ntx)} : items[tc];
     const Inst& I = inst_at<ONE>(insts, one, it.inst);
     bool any = false;
@@ -1104,6 +1117,16 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         int r = 0;
         for (int k = 0; k < w; k++) r += hit[k];
         act[base + r] = it;
+    }
+    if (snap != nullptr && blockIdx.x == 0) {
+        loop_decisions(ctl, ninst, stage);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t* cnt = ctl + CTL_N * ninst + (1 - wlist);
+            snap[CTL_N * ninst + wlist] = *cnt;
+            *cnt = 0;
+        }
+        for (int t = threadIdx.x; t < CTL_N * ninst; t += blockDim.x) snap[t] = ctl[t];
     }
 }
 
@@ -1312,7 +1335,7 @@ void launch_begin(const LaunchSet& L, int stage, hipStream_t st) {
 }
 
 void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStream_t st) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, L.insts, L.ninst, stage, L.ctl, w, snap);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, L.ninst, stage, L.ctl, w, snap);
 }
 
 // the single-solve variant of a kernel when the table holds one solve
@@ -1324,10 +1347,9 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
             hipLaunchKernelGGL(K<false>, grid, block, 0, st, L.insts, L.inst1, __VA_ARGS__); \
     } while (0)
 
-void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st) {
-    if (L.npad <= 0) return;
-    HMHD_LAUNCH1(k_tile_scan, g1((L.npad + kScanTiles - 1) / kScanTiles), dim3(64 * kScanTiles), L.pad, L.npad,
-                 L.act[w], L.act_count + w, L.marks, stamp);
+void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st) {
+    HMHD_LAUNCH1(k_tile_scan, g1(L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1), dim3(64 * kScanTiles), L.pad,
+                 L.npad, L.act[w], L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {

@@ -16,8 +16,11 @@ void launch_begin(const LaunchSet& L, int stage, hipStream_t st);
 // control words copied to `snap` (a device address of pinned host memory)
 void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStream_t st);
 // `w`: which of the two active lists the scan of this iteration writes
-// stamp 0 scans every tile; otherwise only tiles marked with `stamp`
-void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, hipStream_t st);
+// stamp 0 scans every tile; otherwise only tiles marked with `stamp`.
+// With `snap` set the scan also ends the iteration as launch_finish does,
+// except that snap's count slot w gets the length of the list the
+// iteration consumed (list 1 - w)
+void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
 void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st);

@@ -469,9 +469,11 @@ int Engine::snapshot(int s, int w) {
     return hip_check(hipEventRecord(ev_snap_[w], st_), "hipEventRecord");
 }
 
-// wait for snapshot w; its control words become ctl_, and the count of the
-// list that snapshot's iteration wrote becomes act_n_
-int Engine::read_snapshot(int w) {
+// wait for snapshot w; its control words become ctl_, and its count slot
+// (the list the first pass's scan wrote; for a later iteration, the list
+// it consumed) becomes act_n_.  `visited`: the count is the tiles a MOOD
+// iteration visited (profiling counter).
+int Engine::read_snapshot(int w, bool visited) {
     const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 2;
     int rc = hip_check(hipEventSynchronize(ev_snap_[w]), "hipEventSynchronize");
     if (rc) return rc;
@@ -480,7 +482,7 @@ int Engine::read_snapshot(int w) {
     act_n_ = h[CTL_N * n + w];
     prof_collect(false);
     if (prof_mask >> PF_TILES & 1u) {
-        prof_n[PF_TILES] += act_n_;  // active tiles summed over MOOD iterations
+        if (visited) prof_n[PF_TILES] += act_n_;  // active tiles summed over MOOD iterations
         prof_ms[PF_TILES] += 1.0;    // MOOD iterations (host round trips)
     }
     return HMHD_OK;
@@ -512,11 +514,13 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, act, grid, st_);
     launch_sync(L, st_);
-    launch_tile_scan(L, w, stamp_, st_);
+    // the scan also ends the iteration (loop decisions, snapshot)
+    const size_t nctl = (size_t)CTL_N * inst_.size() + 2;
+    launch_tile_scan(L, w, stamp_, s, d_hctl_ + w * nctl, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    return snapshot(s, w);
+    return hip_check(hipEventRecord(ev_snap_[w], st_), "hipEventRecord");
 }
 
 // One RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active
@@ -540,7 +544,7 @@ int Engine::run_stage(int s) {
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, all, L.npad, st_);
     launch_sync(L, st_);
-    launch_tile_scan(L, 0, 0, st_);
+    launch_tile_scan(L, 0, 0, s, nullptr, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
@@ -548,7 +552,7 @@ int Engine::run_stage(int s) {
     auto grid_for = [&](int listed) { return std::min(L.npad, listed + listed / 2 + 16); };
     if ((rc = enqueue_iteration(s, 1, grid_for(est_act_)))) return rc;
     for (int it = 0;; it++) {
-        if ((rc = read_snapshot(it & 1))) return rc;
+        if ((rc = read_snapshot(it & 1, it > 0))) return rc;
         // every continuing iteration lowers at least one scheme value, so
         // the loop ends; the cap only turns a defect into an error
         if (it > 64 * (kMaxR + 1) * 64) {

@@ -94,7 +94,7 @@ class Engine {
     int ensure_stream();
     int commit();
     int snapshot(int s, int w);
-    int read_snapshot(int w);
+    int read_snapshot(int w, bool visited);
     int enqueue_iteration(int s, int it, int grid);
     int run_stage(int s);
     void prof_begin(int fam);
