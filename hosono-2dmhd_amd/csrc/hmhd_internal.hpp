@@ -33,6 +33,7 @@ enum : int {
     CTL_MODE = 3,       // MODE_NEWPASS: reconstruct the rc cells; MODE_INNER: the rec cells
     CTL_PASSES = 4,     // [3] MOOD passes per stage
     CTL_INNER = 7,      // [3] inner admissibility iterations per stage
+    CTL_SYNC = 10,      // this pass ends with the periodic MOOD sync (set before the loop decisions)
     CTL_N = 16
 };
 enum : int { MODE_NEWPASS = 0, MODE_INNER = 1 };

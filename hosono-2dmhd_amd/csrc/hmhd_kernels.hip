@@ -12,7 +12,8 @@
 //              src/timeStep.f90:165,169)
 //   k_lower_*  the order-lowering bookkeeping of src/gridBlock.f90:194-235
 //              written as a gather, so no cell is written by two threads
-//   k_sync_*   periodic MOOD image sync (src/boundaryConditions.f90:189-254)
+//   k_tile_scan  the periodic MOOD sync (src/boundaryConditions.f90:189-254),
+//              the tiles the next iteration needs, and the loop decisions
 //
 // The first pass of a stage runs k_face in FULL mode: every face at the top
 // radius, admissibility checked on the way.  Every later iteration (a new
@@ -22,7 +23,6 @@
 // sides, fluxes getFluxes_ recomputes), those whose inputs changed are
 // recomputed, each side from the radius it was last reconstructed with,
 // and the others only repeat the recomputation's side effects.
-// k_tile_scan lists the tiles the next iteration needs.
 #include <hip/hip_runtime.h>
 
 #include "hmhd_internal.hpp"
@@ -148,6 +148,12 @@ __device__ __forceinline__ Ctl ctl_of(const int32_t* c) {
 // only for instances inside their MOOD loop whose inner admissibility loop
 // has converged
 __device__ __forceinline__ bool pass_ready(const Ctl& c) { return c.run != 0 && c.any_inadm == 0; }
+
+// whether the pass ends with the periodic MOOD sync: a periodic solve whose
+// pass ran (its detect lowering did)
+__device__ __forceinline__ bool sync_due(const Inst& I, const Ctl& c) {
+    return I.bc == BC_PERIODIC && I.active && pass_ready(c);
+}
 
 // The record of a tile's solve: with ONE (a single solve) the kernel
 // argument, so its fields do not wait for the tile's list entry.
@@ -694,8 +700,16 @@ __device__ __forceinline__ void lower_detect_tile(const Inst& I, const Ctl& cw, 
 
 template <bool ONE>
 __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const Inst one,
-                                                      const TileList tl) {
+                                                      const TileList tl, int ninst) {
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
+    // block 0 records, for the sync in the scan launch, whether this pass
+    // ends with one (the scan's loop decisions rewrite the words it reads)
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+            const Inst& I = inst_at<ONE>(insts, one, k);
+            I.ctl[CTL_SYNC] = sync_due(I, ONE ? c1 : ctl_of(I.ctl)) ? 1 : 0;
+        }
+    }
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         lower_detect_tile(I, ONE ? c1 : ctl_of(I.ctl), it);
@@ -710,7 +724,8 @@ __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ i
 // only), a guard row's pair in such a column (y only), and the quad of a
 // guard column and a guard row with their images, whose two x pairs come
 // before its two y pairs as in the sweeps.  Needs N >= 2 NGC, so the guard
-// and image bands are disjoint.  One grid row of blocks per instance.
+// and image bands are disjoint.  Runs in the scan launch, sync_gx blocks
+// per solve.
 // A unit of K cells ends with rc and rcF the OR over the unit and scheme
 // its MIN (for a quad, the x pairs then the y pairs give exactly that).
 // Every value is loaded before any is stored.
@@ -738,10 +753,8 @@ __device__ __forceinline__ void sync_unit(const Inst& I, const size_t (&c)[K]) {
     }
 }
 
-template <bool ONE>
-__global__ __launch_bounds__(256) void k_sync(const Inst* __restrict__ insts, const Inst one) {
-    const Inst& I = inst_at<ONE>(insts, one, blockIdx.y);
-    if (I.bc != BC_PERIODIC || !I.active || !pass_ready(ctl_of(I.ctl))) return;
+// block bx of gx over the sync units of one solve
+__device__ __forceinline__ void sync_block(const Inst& I, int bx, int gx) {
     const int s = I.ngc, g2 = 2 * s, nx = I.nx, ny = I.ny;
     const size_t mx = (size_t)I.mx;
     const int nq = g2 * g2, nxo = g2 * (ny - g2), nyo = g2 * (nx - g2);
@@ -754,7 +767,7 @@ __global__ __launch_bounds__(256) void k_sync(const Inst* __restrict__ insts, co
         j = g < s ? g : ny + g;
         jp = g < s ? j + ny : j - ny;
     };
-    for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < nq + nxo + nyo; u += gridDim.x * blockDim.x) {
+    for (int u = bx * blockDim.x + threadIdx.x; u < nq + nxo + nyo; u += gx * blockDim.x) {
         int i, ip, j, jp;
         if (u < nq) {
             col(u % g2, i, ip);
@@ -1074,13 +1087,21 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
                                                                Item* __restrict__ act, int32_t* __restrict__ act_count,
                                                                const uint32_t* __restrict__ marks, uint32_t stamp,
                                                                int ninst, int stage, int32_t* __restrict__ ctl,
-                                                               int wlist, int32_t* __restrict__ snap) {
+                                                               int wlist, int32_t* __restrict__ snap, int sync_gx) {
     __shared__ int hit[kScanTiles];
     __shared__ int base;
+    const int nsync = sync_gx * ninst;
+    if ((int)blockIdx.x < nsync) {
+        // the periodic MOOD sync, sync_gx blocks per solve
+        const Inst& I = inst_at<ONE>(insts, one, (int)blockIdx.x / sync_gx);
+        if (I.ctl[CTL_SYNC]) sync_block(I, (int)blockIdx.x % sync_gx, sync_gx);
+        return;
+    }
+    const int sb = (int)blockIdx.x - nsync, nsb = (int)gridDim.x - nsync;
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     // a single solve's tiles are its rows of tiles in order, so the tile
     // follows from its index
-    const int t = xcd_tile(blockIdx.x, gridDim.x) * kScanTiles + w;
+    const int t = xcd_tile(sb, nsb) * kScanTiles + w;
     const int tc = t < npad ? t : (npad > 0 ? npad - 1 : 0);  // the list has room for one entry at least
     const Item it = ONE ? Item{0, (int16_t)(tc % one.ntx), (int16_t)(tc / one.ntx)} : items[tc];
     const Inst& I = inst_at<ONE>(insts, one, it.inst);
@@ -1102,6 +1123,23 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         }
 #pragma unroll
         for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
+        // The sync of this launch may not have reached the footprint's
+        // guard / image bands yet: a band cell's rc and rcF are read as the
+        // OR over its sync unit, which is their value once synced.
+        const bool bands = I.ctl[CTL_SYNC] && (x0 < 2 * I.ngc || x0 + W > I.nx || y0 < 2 * I.ngc || y0 + W > I.ny);
+        if (bands && !__ballot(any)) {
+            const int s2 = 2 * I.ngc;
+            for (int c = lane; c < W * W; c += 64) {
+                const int i = x0 + c % W, j = y0 + c / W;
+                if (i < 0 || j < 0 || i >= I.mx || j >= I.my) continue;
+                const bool cb = i < s2 || i >= I.nx, rb = j < s2 || j >= I.ny;
+                if (!(cb || rb)) continue;
+                const int ip = cb ? (i < s2 ? i + I.nx : i - I.nx) : i;
+                const int jp = rb ? (j < s2 ? j + I.ny : j - I.ny) : j;
+                const size_t a = (size_t)j * I.mx + ip, b2 = (size_t)jp * I.mx + i, d = (size_t)jp * I.mx + ip;
+                any |= (I.rc[a] | I.rcF[a] | I.rc[b2] | I.rcF[b2] | I.rc[d] | I.rcF[d]) != 0;
+            }
+        }
     }
     const bool hw = __ballot(any) != 0;
     if (lane == 0) hit[w] = hw ? 1 : 0;
@@ -1118,7 +1156,7 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         for (int k = 0; k < w; k++) r += hit[k];
         act[base + r] = it;
     }
-    if (snap != nullptr && blockIdx.x == 0) {
+    if (snap != nullptr && sb == 0) {
         loop_decisions(ctl, ninst, stage);
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1347,9 +1385,11 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
             hipLaunchKernelGGL(K<false>, grid, block, 0, st, L.insts, L.inst1, __VA_ARGS__); \
     } while (0)
 
-void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st) {
-    HMHD_LAUNCH1(k_tile_scan, g1(L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1), dim3(64 * kScanTiles), L.pad,
-                 L.npad, L.act[w], L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap);
+void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st) {
+    const int nscan = L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1;
+    const int sync_gx = L.sync_gx > 0 ? L.sync_gx : 0;
+    HMHD_LAUNCH1(k_tile_scan, g1(sync_gx * L.ninst + nscan), dim3(64 * kScanTiles), L.pad, L.npad, L.act[w],
+                 L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, sync_gx);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
@@ -1390,16 +1430,7 @@ void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStre
 }
 
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st) {
-    if (grid <= 0) return;
-    HMHD_LAUNCH1(k_lower_detect, g1(grid), dim3(256), tl);
-}
-
-void launch_sync(const LaunchSet& L, hipStream_t st) {
-    if (L.sync_gx <= 0) return;
-    if (L.one)
-        hipLaunchKernelGGL(k_sync<true>, dim3(L.sync_gx, L.ninst), dim3(256), 0, st, L.insts, L.inst1);
-    else
-        hipLaunchKernelGGL(k_sync<false>, dim3(L.sync_gx, L.ninst), dim3(256), 0, st, L.insts, L.inst1);
+    HMHD_LAUNCH1(k_lower_detect, g1(grid > 0 ? grid : 1), dim3(256), tl, L.ninst);
 }
 #undef HMHD_LAUNCH1
 

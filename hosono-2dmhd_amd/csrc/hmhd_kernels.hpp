@@ -20,13 +20,13 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
 // With `snap` set the scan also ends the iteration as launch_finish does,
 // except that snap's count slot w gets the length of the list the
 // iteration consumed (list 1 - w)
-void launch_tile_scan(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st);
+// The periodic MOOD sync runs in the same launch (blocks ahead of the scan).
+void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
 void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st);
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st);
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st);
-void launch_sync(const LaunchSet& L, hipStream_t st);
 void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st);
 void launch_set_dt(Inst* insts, const double* dt, int ninst, hipStream_t st);
 void launch_rotate(Inst* insts, int ninst, hipStream_t st);

@@ -335,7 +335,7 @@ int Engine::commit() {
             tiles(cell, (int)k, h.cfg.nx, h.cfg.ny);
             d.cell_cnt = (int32_t)cell.size() - d.cell_off;
             if (h.cfg.bc == HMHD_BC_PERIODIC) {
-                // units of k_sync: guard quads, x-only and y-only pairs
+                // units of the periodic MOOD sync: guard quads, x-only and y-only pairs
                 const int g2 = 2 * h.ngc;
                 const int units = g2 * g2 + g2 * (h.cfg.ny - g2) + g2 * (h.cfg.nx - g2);
                 max_sync_units = std::max(max_sync_units, units);
@@ -513,10 +513,9 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, act, grid, st_);
-    launch_sync(L, st_);
-    // the scan also ends the iteration (loop decisions, snapshot)
+    // the sync and the scan, which also ends the iteration (loop decisions, snapshot)
     const size_t nctl = (size_t)CTL_N * inst_.size() + 2;
-    launch_tile_scan(L, w, stamp_, s, d_hctl_ + w * nctl, st_);
+    launch_scan_sync(L, w, stamp_, s, d_hctl_ + w * nctl, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
@@ -543,8 +542,7 @@ int Engine::run_stage(int s) {
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, all, L.npad, st_);
-    launch_sync(L, st_);
-    launch_tile_scan(L, 0, 0, s, nullptr, st_);
+    launch_scan_sync(L, 0, 0, s, nullptr, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
