@@ -329,13 +329,16 @@ def plain_form(rt, args):
     win0 = args.breakdown_at if args.steps == 0 else max(0, limit // 2 - args.breakdown_steps // 2)
     win1 = win0 + args.breakdown_steps
     snap0 = snap1 = None
+    w_t0 = w_t1 = None
     t0 = time.perf_counter()
     while t < tmax and steps < limit:
         if steps == win0:
             s.profile(True)
             snap0 = s.profile_read()
+            w_t0 = time.perf_counter()
         elif steps == win1:
             snap1 = s.profile_read()
+            w_t1 = time.perf_counter()
             s.profile(["face_full"])
         dt = rt.valid_time_step(s.cfl_computedt(), t, tmax)
         s.advance_solution(dt)
@@ -349,6 +352,7 @@ def plain_form(rt, args):
     if snap1 is None:
         snap1 = prof
         win1 = steps
+        w_t1 = time.perf_counter()
     breakdown = {k: (snap1[k][0] - snap0[k][0], snap1[k][1] - snap0[k][1]) for k in snap1} if snap0 else {}
 
     zones = args.n * args.n
@@ -381,7 +385,10 @@ def plain_form(rt, args):
         "kernel_breakdown": {"steps": [win0, win1],
                              "ms": {k: v[0] for k, v in breakdown.items()},
                              "launches": {k: v[1] for k, v in breakdown.items()},
-                             "mood_iterations": int(mood_iters), "active_tiles_visited": int(tiles)},
+                             "mood_iterations": int(mood_iters), "active_tiles_visited": int(tiles),
+                             # host wall time of the window's steps, to set against the
+                             # kernel families' event times (the gap is GPU idle time)
+                             "window_wall_ms": (w_t1 - w_t0) * 1e3 if w_t0 is not None else None},
     }
     return line
 

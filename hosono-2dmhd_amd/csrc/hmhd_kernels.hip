@@ -1075,10 +1075,10 @@ __global__ __launch_bounds__(256) void k_finish(int ninst, int stage, int32_t* _
 // with one atomic, so the list count sees one update per block
 constexpr int kScanTiles = 4;
 //
-// With `snap` set, block 0 also ends the MOOD iteration as k_finish does
-// (the decisions read only flags earlier kernels set, and no scan block
-// reads a control word): the decisions, the control words copied to
-// `snap`, and, in snap's count slot w, the length of the list this
+// With `snap` set, one more block ends the MOOD iteration as k_finish does
+// (the decisions read only flags earlier kernels set, and no scan or sync
+// block reads a word they write): the decisions, the control words copied
+// to `snap`, and, in snap's count slot w, the length of the list this
 // iteration consumed (list 1 - w, complete since the previous scan), whose
 // count it then clears for the next scan.
 template <bool ONE>
@@ -1090,14 +1090,26 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
                                                                int wlist, int32_t* __restrict__ snap, int sync_gx) {
     __shared__ int hit[kScanTiles];
     __shared__ int base;
-    const int nsync = sync_gx * ninst;
+    const int nsync = sync_gx * ninst, nfin = snap != nullptr ? 1 : 0;
     if ((int)blockIdx.x < nsync) {
         // the periodic MOOD sync, sync_gx blocks per solve
         const Inst& I = inst_at<ONE>(insts, one, (int)blockIdx.x / sync_gx);
         if (I.ctl[CTL_SYNC]) sync_block(I, (int)blockIdx.x % sync_gx, sync_gx);
         return;
     }
-    const int sb = (int)blockIdx.x - nsync, nsb = (int)gridDim.x - nsync;
+    if ((int)blockIdx.x == nsync && nfin) {
+        // the end of the iteration, in a block of its own
+        loop_decisions(ctl, ninst, stage);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t* cnt = ctl + CTL_N * ninst + (1 - wlist);
+            snap[CTL_N * ninst + wlist] = *cnt;
+            *cnt = 0;
+        }
+        for (int t = threadIdx.x; t < CTL_N * ninst; t += blockDim.x) snap[t] = ctl[t];
+        return;
+    }
+    const int sb = (int)blockIdx.x - nsync - nfin, nsb = (int)gridDim.x - nsync - nfin;
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     // a single solve's tiles are its rows of tiles in order, so the tile
     // follows from its index
@@ -1155,16 +1167,6 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         int r = 0;
         for (int k = 0; k < w; k++) r += hit[k];
         act[base + r] = it;
-    }
-    if (snap != nullptr && sb == 0) {
-        loop_decisions(ctl, ninst, stage);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int32_t* cnt = ctl + CTL_N * ninst + (1 - wlist);
-            snap[CTL_N * ninst + wlist] = *cnt;
-            *cnt = 0;
-        }
-        for (int t = threadIdx.x; t < CTL_N * ninst; t += blockDim.x) snap[t] = ctl[t];
     }
 }
 
@@ -1388,7 +1390,8 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
 void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st) {
     const int nscan = L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1;
     const int sync_gx = L.sync_gx > 0 ? L.sync_gx : 0;
-    HMHD_LAUNCH1(k_tile_scan, g1(sync_gx * L.ninst + nscan), dim3(64 * kScanTiles), L.pad, L.npad, L.act[w],
+    HMHD_LAUNCH1(k_tile_scan, g1(sync_gx * L.ninst + (snap ? 1 : 0) + nscan), dim3(64 * kScanTiles), L.pad, L.npad,
+                 L.act[w],
                  L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, sync_gx);
 }
 
