@@ -680,7 +680,7 @@ __global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ in
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         lower_inner_tile(I, ONE ? c1 : ctl_of(I.ctl), it, marks, stamp);
-   }, c1.run | c1.any_inadm | c1.mode);
+    }, c1.run | c1.any_inadm | c1.mode);
 }
 
 // after detect: rc = cross(low), rcF = diamond(low), scheme lowered
@@ -713,7 +713,7 @@ __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ i
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         lower_detect_tile(I, ONE ? c1 : ctl_of(I.ctl), it);
-   }, c1.run | c1.any_inadm | c1.mode);
+    }, c1.run | c1.any_inadm | c1.mode);
 }
 
 // Periodic MOOD sync (src/boundaryConditions.f90:205-252) in one launch.
@@ -1009,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         update_tile(I, ONE ? c1 : ctl_of(I.ctl), it, stage);
-   }, c1.run | c1.any_inadm | c1.mode);
+    }, c1.run | c1.any_inadm | c1.mode);
 }
 
 // Start of a stage's MOOD loop, one thread per instance (inactive
