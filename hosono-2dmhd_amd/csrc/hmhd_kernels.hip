@@ -1058,12 +1058,23 @@ __device__ __forceinline__ void loop_decisions(int32_t* __restrict__ ctl, int ni
     }
 }
 
+// The host polls a snapshot's sequence word (after the control words and
+// the two counts): it is stored last, behind a system-scope fence.
+__device__ __forceinline__ void publish_snapshot(int32_t* __restrict__ snap, int ninst, uint32_t seq) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(snap + CTL_N * ninst + 2, (int32_t)seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_finish(int ninst, int stage, int32_t* __restrict__ ctl, int w,
-                                                int32_t* __restrict__ snap) {
+                                                int32_t* __restrict__ snap, uint32_t seq) {
     loop_decisions(ctl, ninst, stage);
     if (threadIdx.x == 0) ctl[CTL_N * ninst + (1 - w)] = 0;
     __syncthreads();
     for (int t = threadIdx.x; t < CTL_N * ninst + 2; t += blockDim.x) snap[t] = ctl[t];
+    publish_snapshot(snap, ninst, seq);
 }
 
 // Active tiles of the next MOOD iteration: every quantity a pass writes
@@ -1080,14 +1091,15 @@ constexpr int kScanTiles = 4;
 // block reads a word they write): the decisions, the control words copied
 // to `snap`, and, in snap's count slot w, the length of the list this
 // iteration consumed (list 1 - w, complete since the previous scan), whose
-// count it then clears for the next scan.
+// count it then clears for the next scan; the sequence word `seq` last.
 template <bool ONE>
 __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
                                                                const Item* __restrict__ items, int npad,
                                                                Item* __restrict__ act, int32_t* __restrict__ act_count,
                                                                const uint32_t* __restrict__ marks, uint32_t stamp,
                                                                int ninst, int stage, int32_t* __restrict__ ctl,
-                                                               int wlist, int32_t* __restrict__ snap, int sync_gx) {
+                                                               int wlist, int32_t* __restrict__ snap, uint32_t seq,
+                                                               int sync_gx) {
     __shared__ int hit[kScanTiles];
     __shared__ int base;
     const int nsync = sync_gx * ninst, nfin = snap != nullptr ? 1 : 0;
@@ -1107,6 +1119,7 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
             *cnt = 0;
         }
         for (int t = threadIdx.x; t < CTL_N * ninst; t += blockDim.x) snap[t] = ctl[t];
+        publish_snapshot(snap, ninst, seq);
         return;
     }
     const int sb = (int)blockIdx.x - nsync - nfin, nsb = (int)gridDim.x - nsync - nfin;
@@ -1374,8 +1387,8 @@ void launch_begin(const LaunchSet& L, int stage, hipStream_t st) {
     hipLaunchKernelGGL(k_begin, dim3((L.ninst + 63) / 64), dim3(64), 0, st, L.insts, L.ninst, stage, L.act_count);
 }
 
-void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStream_t st) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, L.ninst, stage, L.ctl, w, snap);
+void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t seq, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, L.ninst, stage, L.ctl, w, snap, seq);
 }
 
 // the single-solve variant of a kernel when the table holds one solve
@@ -1387,12 +1400,13 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStrea
             hipLaunchKernelGGL(K<false>, grid, block, 0, st, L.insts, L.inst1, __VA_ARGS__); \
     } while (0)
 
-void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st) {
+void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
+                      hipStream_t st) {
     const int nscan = L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1;
     const int sync_gx = L.sync_gx > 0 ? L.sync_gx : 0;
     HMHD_LAUNCH1(k_tile_scan, g1(sync_gx * L.ninst + (snap ? 1 : 0) + nscan), dim3(64 * kScanTiles), L.pad, L.npad,
                  L.act[w],
-                 L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, sync_gx);
+                 L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, seq, sync_gx);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {

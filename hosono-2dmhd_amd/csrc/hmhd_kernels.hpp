@@ -13,15 +13,17 @@ void launch_stage_init(const LaunchSet& L, hipStream_t st);
 // start of a stage's MOOD loop (the first pass's scan writes list 0)
 void launch_begin(const LaunchSet& L, int stage, hipStream_t st);
 // end of a MOOD iteration whose scan wrote list w: loop decisions, and the
-// control words copied to `snap` (a device address of pinned host memory)
-void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, hipStream_t st);
+// control words copied to `snap` (a device address of pinned host memory),
+// whose sequence word is then set to `seq`
+void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t seq, hipStream_t st);
 // `w`: which of the two active lists the scan of this iteration writes
 // stamp 0 scans every tile; otherwise only tiles marked with `stamp`.
 // With `snap` set the scan also ends the iteration as launch_finish does,
 // except that snap's count slot w gets the length of the list the
 // iteration consumed (list 1 - w)
 // The periodic MOOD sync runs in the same launch (blocks ahead of the scan).
-void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, hipStream_t st);
+void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
+                      hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
 void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st);

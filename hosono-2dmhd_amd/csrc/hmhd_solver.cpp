@@ -5,6 +5,7 @@
 #include "hmhd_solver.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -72,10 +73,7 @@ int Engine::hip_check(hipError_t e, const char* what) {
 
 int Engine::ensure_stream() {
     if (st_) return HMHD_OK;
-    int rc = hip_check(hipStreamCreate(&st_), "hipStreamCreate");
-    for (int w = 0; w < 2 && !rc; w++)
-        rc = hip_check(hipEventCreateWithFlags(&ev_snap_[w], hipEventDisableTiming), "hipEventCreate");
-    return rc;
+    return hip_check(hipStreamCreate(&st_), "hipStreamCreate");
 }
 
 Engine::~Engine() {
@@ -92,8 +90,6 @@ Engine::~Engine() {
     for (void* p : host)
         if (p) (void)hipHostFree(p);
     for (auto e : ev_all_) (void)hipEventDestroy(e);
-    for (auto e : ev_snap_)
-        if (e) (void)hipEventDestroy(e);
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -262,7 +258,7 @@ int Engine::set_active(int64_t id, bool on) {
 }
 
 template <class T>
-int Engine::grow(T** p, size_t& cap, size_t need, bool host) {
+int Engine::grow(T** p, size_t& cap, size_t need, bool host, unsigned host_flags) {
     if (need <= cap && *p) return HMHD_OK;
     size_t n = std::max<size_t>(need, 2 * cap);
     n = std::max<size_t>(n, 16);
@@ -273,7 +269,7 @@ int Engine::grow(T** p, size_t& cap, size_t need, bool host) {
             (void)hipFree(*p);
         *p = nullptr;
     }
-    int rc = host ? hip_check(hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * n), "hipHostMalloc")
+    int rc = host ? hip_check(hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * n, host_flags), "hipHostMalloc")
                   : hip_check(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * n), "hipMalloc");
     if (rc) return rc;
     cap = n;
@@ -297,7 +293,8 @@ int Engine::commit() {
     const size_t nctl = (size_t)CTL_N * n + 4;
     if ((rc = grow(&d_insts_, cap_insts_, n, false))) return rc;
     if ((rc = grow(&d_ctl_, cap_ctl_, nctl, false))) return rc;
-    if ((rc = grow(&h_ctl_, cap_hctl_, 2 * nctl, true))) return rc;
+    // snapshots are polled while the stream runs: coherent (fine-grained) memory
+    if ((rc = grow(&h_ctl_, cap_hctl_, 2 * nctl, true, hipHostMallocCoherent | hipHostMallocMapped))) return rc;
     if ((rc = hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_hctl_), h_ctl_, 0),
                         "hipHostGetDevicePointer")))
         return rc;
@@ -462,11 +459,15 @@ void Engine::prof_collect(bool wait_all) {
 // decisions, and the control words and both list counts written to host
 // slot w, ready when event w has fired
 int Engine::snapshot(int s, int w) {
-    const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 2;
-    launch_finish(L, s, w, d_hctl_ + w * nctl, st_);
-    int rc = hip_check(hipGetLastError(), "kernel launch");
-    if (rc) return rc;
-    return hip_check(hipEventRecord(ev_snap_[w], st_), "hipEventRecord");
+    launch_finish(L, s, w, d_hctl_ + w * snap_stride(), next_seq(w), st_);
+    return hip_check(hipGetLastError(), "kernel launch");
+}
+
+// the sequence number the next snapshot into slot w carries (never 0)
+uint32_t Engine::next_seq(int w) {
+    if (++snap_seq_ == 0) snap_seq_ = 1;
+    snap_want_[w] = snap_seq_;
+    return snap_seq_;
 }
 
 // wait for snapshot w; its control words become ctl_, and its count slot
@@ -474,10 +475,25 @@ int Engine::snapshot(int s, int w) {
 // it consumed) becomes act_n_.  `visited`: the count is the tiles a MOOD
 // iteration visited (profiling counter).
 int Engine::read_snapshot(int w, bool visited) {
-    const size_t n = inst_.size(), nctl = (size_t)CTL_N * n + 2;
-    int rc = hip_check(hipEventSynchronize(ev_snap_[w]), "hipEventSynchronize");
-    if (rc) return rc;
-    const int32_t* h = h_ctl_ + w * nctl;
+    const size_t n = inst_.size();
+    const int32_t* h = h_ctl_ + w * snap_stride();
+    // The snapshot's block stores its sequence number last, after a
+    // system-scope fence; wait for it in the pinned memory itself (no event
+    // in the stream).  An idle stream without it is an error, not a hang.
+    const volatile int32_t* seq = h + CTL_N * n + 2;
+    const int32_t want = (int32_t)snap_want_[w];
+    for (unsigned k = 1; *seq != want; k++) {
+        if (k % 1024 == 0) {
+            const hipError_t e = hipStreamQuery(st_);
+            if (e == hipSuccess && *seq != want) {
+                err = "MOOD iteration snapshot missing";
+                return HMHD_ERR_DEVICE;
+            }
+            if (e != hipSuccess && e != hipErrorNotReady) return hip_check(e, "hipStreamQuery");
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
     std::memcpy(ctl_.data(), h, sizeof(int32_t) * CTL_N * n);
     act_n_ = h[CTL_N * n + w];
     prof_collect(false);
@@ -514,12 +530,9 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, act, grid, st_);
     // the sync and the scan, which also ends the iteration (loop decisions, snapshot)
-    const size_t nctl = (size_t)CTL_N * inst_.size() + 2;
-    launch_scan_sync(L, w, stamp_, s, d_hctl_ + w * nctl, st_);
+    launch_scan_sync(L, w, stamp_, s, d_hctl_ + w * snap_stride(), next_seq(w), st_);
     prof_end(PF_FLAGS);
-    int rc = hip_check(hipGetLastError(), "kernel launch");
-    if (rc) return rc;
-    return hip_check(hipEventRecord(ev_snap_[w], st_), "hipEventRecord");
+    return hip_check(hipGetLastError(), "kernel launch");
 }
 
 // One RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active
@@ -542,7 +555,7 @@ int Engine::run_stage(int s) {
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, all, L.npad, st_);
-    launch_scan_sync(L, 0, 0, s, nullptr, st_);
+    launch_scan_sync(L, 0, 0, s, nullptr, 0, st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;

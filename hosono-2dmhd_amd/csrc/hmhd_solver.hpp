@@ -102,7 +102,7 @@ class Engine {
     void prof_collect(bool wait_all);
     void launch_face_groups(TileList tl, int grid, bool full, int stage);
     template <class T>
-    int grow(T** p, size_t& cap, size_t need, bool host);
+    int grow(T** p, size_t& cap, size_t need, bool host, unsigned host_flags = hipHostMallocDefault);
 
     std::vector<InstHost> inst_;  // dense order = device table order
     std::unordered_map<int64_t, size_t> index_;
@@ -119,7 +119,10 @@ class Engine {
     int32_t* h_ctl_ = nullptr;  // pinned, two snapshot slots
     int32_t* d_hctl_ = nullptr;  // its device address
     size_t cap_hctl_ = 0;
-    hipEvent_t ev_snap_[2] = {nullptr, nullptr};
+    // snapshot slot layout: CTL_N words per solve, two list counts, the sequence number
+    size_t snap_stride() const { return (size_t)CTL_N * inst_.size() + 4; }
+    uint32_t next_seq(int w);
+    uint32_t snap_seq_ = 0, snap_want_[2] = {0, 0};
     unsigned long long* d_cfl_ = nullptr;  // two maxima per solve
     size_t cap_cfl_ = 0;
     unsigned long long* h_cfl_ = nullptr;
