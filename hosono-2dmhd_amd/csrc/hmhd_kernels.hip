@@ -18,8 +18,9 @@
 // The first pass of a stage runs k_face in FULL mode: every face at the top
 // radius, admissibility checked on the way.  Every later iteration (a new
 // MOOD pass, or another round of the inner admissibility loop when a
-// reconstructed state was inadmissible) runs k_lower_inner and k_face on the
-// active tiles only: of the faces the reference would rebuild (flagged
+// reconstructed state was inadmissible) runs k_face on the active tiles
+// only (the inner round's lowering ran in the scan launch before it): of
+// the faces the reference would rebuild (flagged
 // sides, fluxes getFluxes_ recomputes), those whose inputs changed are
 // recomputed, each side from the radius it was last reconstructed with,
 // and the others only repeat the recomputation's side effects.
@@ -451,6 +452,29 @@ __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set
     }
 }
 
+// Tiles whose flags this iteration may change, for the scan that ends it:
+// within one tile of an active tile (every kernel writes flags of its own
+// tile's cells, the face kernel also the east / north sides of the cells
+// west / south of it), and, for a periodic solve, around the images of the
+// tile by +-N in x and y, where the MOOD sync copies guard-band flags.
+// A tile outside this set had no flag in its footprint after the previous
+// iteration and gains none in this one, so its scan would come out empty.
+__device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, uint32_t* __restrict__ marks,
+                                                uint32_t stamp) {
+    const int t = threadIdx.x;
+    if (t >= 9 || (t > 0 && I.bc != BC_PERIODIC)) return;
+    const int sx = (t % 3 == 1) ? I.nx : (t % 3 == 2 ? -I.nx : 0);
+    const int sy = (t / 3 == 1) ? I.ny : (t / 3 == 2 ? -I.ny : 0);
+    const int nty = (I.my + kCellT - 1) / kCellT;
+    const int x0 = it.tx * kCellT + sx - kCellT, x1 = it.tx * kCellT + sx + 2 * kCellT - 1;
+    const int y0 = it.ty * kCellT + sy - kCellT, y1 = it.ty * kCellT + sy + 2 * kCellT - 1;
+    if (x1 < 0 || y1 < 0 || x0 >= I.mx || y0 >= I.my) return;
+    const int ta = x0 < 0 ? 0 : x0 / kCellT, tb = x1 >= I.mx ? I.ntx - 1 : x1 / kCellT;
+    const int ua = y0 < 0 ? 0 : y0 / kCellT, ub = y1 >= I.my ? nty - 1 : y1 / kCellT;
+    for (int u = ua; u <= ub; u++)
+        for (int v = ta; v <= tb; v++) marks[I.tile_off + u * I.ntx + v] = stamp;
+}
+
 // A later MOOD iteration on one tile.  Every face the reference's pass
 // would touch -- a side to re-reconstruct and check, a flux to recompute
 // -- is classified first.  A face whose sides would be rebuilt at the radii
@@ -464,11 +488,13 @@ __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set
 template <int R>
 __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int stage,
                                                double* __restrict__ sU, uint32_t* __restrict__ lst,
-                                               int* __restrict__ lcnt) {
+                                               int* __restrict__ lcnt, uint32_t* __restrict__ marks,
+                                               uint32_t stamp) {
     constexpr int TX = kFaceTX, TY = kFaceTY, NC = TX * TY, H = R + 1, NT = face_threads(R, false);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
     if (I.R != R || !cw.run) return;
+    mark_candidates(I, it, marks, stamp);
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
     const int i0 = it.tx * TX, j0 = it.ty * TY;
     const size_t fplane = (size_t)mx * I.my;
@@ -557,9 +583,11 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
 
+// Later passes also mark the scan candidates of the iteration (`stamp`).
 template <int R, bool FULL, bool ONE>
 __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
-    const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage) {
+    const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage, uint32_t* __restrict__ marks,
+    uint32_t stamp) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
     __shared__ uint32_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
     __shared__ int lcnt[2];
@@ -572,7 +600,7 @@ __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES
         if (FULL)
             face_tile_full<R>(I, it, stage, sU);
         else
-            face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
+            face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt, marks, stamp);
     }, c1.run | c1.mode);
 }
 
@@ -621,38 +649,12 @@ __device__ __forceinline__ uint8_t inadm_cell(const Inst& I, size_t idx, size_t 
     return I.inadm[idx] | I.inadm[fplane + idx] | I.inadm[2 * fplane + idx] | I.inadm[3 * fplane + idx];
 }
 
-// lowering of every inadmissible cell, as a gather (src/reconstruct.f90:146-171
-// with src/gridBlock.f90:194-235): rc |= cross, rcF |= diamond, rec = cross
-// Tiles whose flags this iteration may change, for the scan that ends it:
-// within one tile of an active tile (every kernel writes flags of its own
-// tile's cells, the face kernel also the east / north sides of the cells
-// west / south of it), and, for a periodic solve, around the images of the
-// tile by +-N in x and y, where the MOOD sync copies guard-band flags.
-// A tile outside this set had no flag in its footprint after the previous
-// iteration and gains none in this one, so its scan would come out empty.
-__device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, uint32_t* __restrict__ marks,
-                                                uint32_t stamp) {
-    const int t = threadIdx.x;
-    if (t >= 9 || (t > 0 && I.bc != BC_PERIODIC)) return;
-    const int sx = (t % 3 == 1) ? I.nx : (t % 3 == 2 ? -I.nx : 0);
-    const int sy = (t / 3 == 1) ? I.ny : (t / 3 == 2 ? -I.ny : 0);
-    const int nty = (I.my + kCellT - 1) / kCellT;
-    const int x0 = it.tx * kCellT + sx - kCellT, x1 = it.tx * kCellT + sx + 2 * kCellT - 1;
-    const int y0 = it.ty * kCellT + sy - kCellT, y1 = it.ty * kCellT + sy + 2 * kCellT - 1;
-    if (x1 < 0 || y1 < 0 || x0 >= I.mx || y0 >= I.my) return;
-    const int ta = x0 < 0 ? 0 : x0 / kCellT, tb = x1 >= I.mx ? I.ntx - 1 : x1 / kCellT;
-    const int ua = y0 < 0 ? 0 : y0 / kCellT, ub = y1 >= I.my ? nty - 1 : y1 / kCellT;
-    for (int u = ua; u <= ub; u++)
-        for (int v = ta; v <= tb; v++) marks[I.tile_off + u * I.ntx + v] = stamp;
-}
 
-__device__ __forceinline__ void lower_inner_tile(const Inst& I, const Ctl& cw, const Item it,
-                                                 uint32_t* __restrict__ marks, uint32_t stamp) {
-    if (!cw.run) return;
-    mark_candidates(I, it, marks, stamp);
-    if (cw.mode != MODE_INNER) return;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
-    if (i >= I.mx || j >= I.my) return;
+// The inner-loop lowering of one cell (src/reconstruct.f90:146-171 with
+// src/gridBlock.f90:194-235) as a gather: an inadmissible cell lowers its
+// scheme; rc |= cross, rcF |= diamond, rec = cross of the inadmissible
+// cells.
+__device__ __forceinline__ void lower_cell(const Inst& I, int i, int j) {
     const size_t idx = (size_t)j * I.mx + i, fp = (size_t)I.mx * I.my;
     // every load before any store (the byte planes may alias as far as the
     // compiler knows)
@@ -670,17 +672,6 @@ __device__ __forceinline__ void lower_inner_tile(const Inst& I, const Ctl& cw, c
     I.rc[idx] = rc | cross;
     I.rcF[idx] = rcf | diam;
     I.rec[idx] = cross;
-}
-
-template <bool ONE>
-__global__ __launch_bounds__(256) void k_lower_inner(const Inst* __restrict__ insts, const Inst one,
-                                                     const TileList tl, uint32_t* __restrict__ marks,
-                                                     uint32_t stamp) {
-    const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
-    for_tiles(tl, [&](const Item it) {
-        const Inst& I = inst_at<ONE>(insts, one, it.inst);
-        lower_inner_tile(I, ONE ? c1 : ctl_of(I.ctl), it, marks, stamp);
-    }, c1.run | c1.any_inadm | c1.mode);
 }
 
 // after detect: rc = cross(low), rcF = diamond(low), scheme lowered
@@ -702,12 +693,16 @@ template <bool ONE>
 __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const Inst one,
                                                       const TileList tl, int ninst) {
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
-    // block 0 records, for the sync in the scan launch, whether this pass
-    // ends with one (the scan's loop decisions rewrite the words it reads)
+    // block 0 records, for the scan launch, whether this pass ends with the
+    // periodic sync and whether the next iteration is an inner-loop round
+    // whose lowering that launch does (its loop decisions rewrite the words
+    // these come from)
     if (blockIdx.x == 0) {
         for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
             const Inst& I = inst_at<ONE>(insts, one, k);
-            I.ctl[CTL_SYNC] = sync_due(I, ONE ? c1 : ctl_of(I.ctl)) ? 1 : 0;
+            const Ctl c = ONE ? c1 : ctl_of(I.ctl);
+            I.ctl[CTL_SYNC] = sync_due(I, c) ? 1 : 0;
+            I.ctl[CTL_LOWER] = c.run != 0 && c.any_inadm != 0 ? 1 : 0;
         }
     }
     for_tiles(tl, [&](const Item it) {
@@ -1167,6 +1162,19 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         }
     }
     const bool hw = __ballot(any) != 0;
+    // The lowering of the next (inner-loop) iteration, on the tiles it will
+    // visit.  Every cell it changes lies within two cells of an inadmissible
+    // cell, so its tile is listed whatever this scan reads.  Other waves may
+    // read flags it has just raised and list a tile the plain order would
+    // not; such a tile holds no flag of its own and its kernels change
+    // nothing.
+    if (hw && I.ctl[CTL_LOWER]) {
+        const int ci0 = it.tx * kCellT, cj0 = it.ty * kCellT;
+        for (int c = lane; c < kCellT * kCellT; c += 64) {
+            const int i = ci0 + c % kCellT, j = cj0 + c / kCellT;
+            if (i < I.mx && j < I.my) lower_cell(I, i, j);
+        }
+    }
     if (lane == 0) hit[w] = hw ? 1 : 0;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1409,16 +1417,17 @@ void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int3
                  L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, seq, sync_gx);
 }
 
-void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
+void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, uint32_t stamp,
+                 hipStream_t st) {
     if (grid <= 0) return;
     const dim3 g = g1(grid);
 #define HMHD_FACE(RR, ONE)                                                                                   \
     if (full)                                                                                                \
         hipLaunchKernelGGL((k_face<RR, true, ONE>), g, dim3(face_threads(RR, true)), 0, st, L.insts, L.inst1, \
-                           tl, stage);                                                                       \
+                           tl, stage, L.marks, stamp);                                                       \
     else                                                                                                     \
         hipLaunchKernelGGL((k_face<RR, false, ONE>), g, dim3(face_threads(RR, false)), 0, st, L.insts,        \
-                           L.inst1, tl, stage);
+                           L.inst1, tl, stage, L.marks, stamp);
 #define HMHD_FACE_R(ONE)   \
     if (R == 1) {          \
         HMHD_FACE(1, ONE)  \
@@ -1434,11 +1443,6 @@ void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, in
     }
 #undef HMHD_FACE_R
 #undef HMHD_FACE
-}
-
-void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st) {
-    if (grid <= 0) return;
-    HMHD_LAUNCH1(k_lower_inner, g1(grid), dim3(256), tl, L.marks, stamp);
 }
 
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st) {

@@ -25,8 +25,9 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t
 void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
                       hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
-void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
-void launch_lower_inner(const LaunchSet& L, TileList tl, int grid, uint32_t stamp, hipStream_t st);
+// later passes also mark the iteration's scan candidates with `stamp`
+void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, uint32_t stamp,
+                 hipStream_t st);
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st);
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st);
 void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st);

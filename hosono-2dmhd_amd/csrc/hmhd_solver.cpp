@@ -505,9 +505,9 @@ int Engine::read_snapshot(int w, bool visited) {
 }
 
 // templated kernels run once per GP radius present (tiles of other radii return)
-void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
+void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage, uint32_t stamp) {
     for (int R = 1; R <= kMaxR; R++)
-        if (has_R_[R]) launch_face(L, tl, grid, R, full, stage, st_);
+        if (has_R_[R]) launch_face(L, tl, grid, R, full, stage, stamp, st_);
 }
 
 // MOOD iteration it >= 1 of stage s: the active tiles listed by iteration
@@ -518,11 +518,9 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     const TileList act{L.act[r], L.act_count + r, 0};
     // candidate stamps are never 0 (0 asks the scan for every tile)
     if (++stamp_ == 0) stamp_ = 1;
-    prof_begin(PF_RECON);
-    launch_lower_inner(L, act, grid, stamp_, st_);
-    prof_end(PF_RECON);
+    // (an inner-loop round's lowering ran in the previous scan launch)
     prof_begin(PF_FACE_FLUX);
-    launch_face_groups(act, grid, false, s);
+    launch_face_groups(act, grid, false, s, stamp_);
     prof_end(PF_FACE_FLUX);
     prof_begin(PF_UPDATE);
     launch_update(L, act, grid, s, st_);
@@ -548,7 +546,7 @@ int Engine::run_stage(int s) {
     launch_begin(L, s, st_);
     launch_stage_init(L, st_);
     prof_begin(PF_FACE_FULL);
-    launch_face_groups(all, L.npad, true, s);
+    launch_face_groups(all, L.npad, true, s, 0);
     prof_end(PF_FACE_FULL);
     prof_begin(PF_UPDATE);
     launch_update(L, all, L.npad, s, st_);

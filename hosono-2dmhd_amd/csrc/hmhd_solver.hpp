@@ -47,7 +47,7 @@ enum ProfFamily : int {
     PF_FACE_FULL = 0,
     PF_FACE_FLUX,
     PF_UPDATE,
-    PF_RECON,
+    PF_RECON,  // no launches (the inner-loop lowering runs in the scan launch)
     PF_FLAGS,
     PF_CFL,
     PF_TILES,  // counters, not times: MOOD iterations / active tiles
@@ -100,7 +100,7 @@ class Engine {
     void prof_begin(int fam);
     void prof_end(int fam);
     void prof_collect(bool wait_all);
-    void launch_face_groups(TileList tl, int grid, bool full, int stage);
+    void launch_face_groups(TileList tl, int grid, bool full, int stage, uint32_t stamp);
     template <class T>
     int grow(T** p, size_t& cap, size_t need, bool host, unsigned host_flags = hipHostMallocDefault);
 

@@ -115,9 +115,11 @@ int hmhd_stage_schemes(const hmhd_t* h, int32_t* out);
  * HMHD_PROFILE_ALL for every one, 0 to stop): each launch of a selected
  * family is bracketed by events.  hmhd_profile_read returns, per family,
  * the total milliseconds and launch counts since the last reset (order:
- * face_full, face_flux, update, recon, flags, cfl, tiles; "tiles" holds
- * counts, not times: ms[6] counts MOOD iterations (host round trips) and
- * launches[6] the active tiles they visited). */
+ * face_full, face_flux, update, recon, flags, cfl, tiles; "recon" times
+ * nothing (the inner-loop lowering runs in the "flags" family's scan
+ * launch) and keeps its place in the order; "tiles" holds counts, not
+ * times: ms[6] counts MOOD iterations (host round trips) and launches[6]
+ * the active tiles they visited). */
 #define HMHD_PROFILE_ALL 0x7f
 int hmhd_profile(hmhd_t* h, int32_t families);
 int hmhd_profile_read(hmhd_t* h, double* ms, int64_t* launches, int32_t reset);

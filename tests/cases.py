@@ -35,6 +35,14 @@ def make(name, n=None, R=None):
         g = ics.Grid(n, n, -0.5, 0.5, -0.5, 0.5, R=R)
         U, V = ics.mhd_blast(g)
         cfg = O.Config(n, n, -0.5, 0.5, -0.5, 0.5, R=R, bc=O.BC_OUTFLOW)
+    elif name == "blast_periodic":  # MHD blast centred on a corner of a periodic domain:
+        n = n or 48                   # inner-loop rounds across the periodic guard bands
+        R = R or 2
+        g = ics.Grid(n, n, 0.0, 1.0, 0.0, 1.0, R=R)
+        U, V = ics.mhd_blast(g)
+        ics.bc_fill(U, g, ics.BC_PERIODIC)
+        ics.bc_fill(V, g, ics.BC_PERIODIC)
+        cfg = O.Config(n, n, 0.0, 1.0, 0.0, 1.0, R=R)
     elif name == "blast_rect":
         R = R or 2
         g = ics.Grid(64, 40, -0.5, 0.5, -0.3, 0.3, R=R)

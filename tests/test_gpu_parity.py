@@ -94,6 +94,7 @@ def test_download_before_any_step_returns_upload():
     ("sedov_ref", 6),       # R = 3 hydro
     ("ot_rect", 8),         # nx != ny, dx != dy, periodic
     ("blast_rect", 8),      # nx != ny, outflow
+    ("blast_periodic", 8),  # inner-loop rounds across periodic guard bands
 ])
 def test_steps_bitwise(name, steps):
     cfg, g, orc, gpu = _pair(name)
