@@ -97,7 +97,7 @@ struct TileList {
 
 struct LaunchSet {
     const Inst* insts;
-    int32_t* ctl;      // CTL_N words per instance, then the two active-list counts
+    int32_t* ctl;      // CTL_N words per instance, then the list counts (act_count), then npad
     const Item* pad;   // every tile of the padded grids
     int npad;
     int ninst;
@@ -105,7 +105,11 @@ struct LaunchSet {
     const Item* cell;  // interior tiles for CFL
     int ncell;
     Item* act[2];        // active tiles, written by alternate MOOD iterations
-    int32_t* act_count;  // [2] their counts
+    // the face kernel's tiles of the same iteration: the active tiles, or,
+    // before an inner-loop round, those with an inadmissible cell in reach
+    Item* actf[2];
+    // [4]: list r's counts as one 64-bit word, act[r]'s at 2 r, actf[r]'s at 2 r + 1
+    int32_t* act_count;
     const int32_t* npad_dev;  // npad, on the device (the length of `pad` as a TileList)
     uint32_t* marks;     // per tile: stamp of the last iteration that may change its flags
     unsigned long long* cflpart;  // two partial maxima per CFL block

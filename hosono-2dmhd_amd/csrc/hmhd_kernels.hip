@@ -488,13 +488,12 @@ __device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, ui
 template <int R>
 __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int stage,
                                                double* __restrict__ sU, uint32_t* __restrict__ lst,
-                                               int* __restrict__ lcnt, uint32_t* __restrict__ marks,
-                                               uint32_t stamp) {
+                                               int* __restrict__ lcnt) {
     constexpr int TX = kFaceTX, TY = kFaceTY, NC = TX * TY, H = R + 1, NT = face_threads(R, false);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
     if (I.R != R || !cw.run) return;
-    mark_candidates(I, it, marks, stamp);
+    const bool inner = cw.mode == MODE_INNER;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
     const int i0 = it.tx * TX, j0 = it.ty * TY;
     const size_t fplane = (size_t)mx * I.my;
@@ -526,7 +525,11 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const bool cellL = d == 0 ? (i >= s - 1 && j <= ey + 2) : (j >= s - 1 && i <= ex + 2);
         const bool recR = cellR && flagR, recL = cellL && flagL;
         const bool fl = i >= s - 1 && i <= ex + 2 && j >= s - 1 && j <= ey + 2 && (rcR || rcL);
-        if (!(recL || recR || fl)) continue;
+        // In an inner-loop round a face with no side to re-reconstruct has
+        // nothing to do: no update has run since the face kernel before,
+        // which left its flux and FLB current, and its radius can only have
+        // changed with a side's cell lowered, which flags that side.
+        if (inner ? !(recL || recR) : !(recL || recR || fl)) continue;
         const int rr_now = sc < sn ? sc : sn;
         const bool keepL = !recL || rrL == rr_now;
         const bool keepR = !recR || rrR == rr_now;
@@ -583,11 +586,9 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
 }
 
-// Later passes also mark the scan candidates of the iteration (`stamp`).
 template <int R, bool FULL, bool ONE>
 __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
-    const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage, uint32_t* __restrict__ marks,
-    uint32_t stamp) {
+    const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
     __shared__ uint32_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
     __shared__ int lcnt[2];
@@ -600,7 +601,7 @@ __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES
         if (FULL)
             face_tile_full<R>(I, it, stage, sU);
         else
-            face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt, marks, stamp);
+            face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
     }, c1.run | c1.mode);
 }
 
@@ -999,11 +1000,15 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
 
 template <bool ONE>
 __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
-                                                int stage) {
+                                                int stage, uint32_t* __restrict__ marks, uint32_t stamp) {
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
-        update_tile(I, ONE ? c1 : ctl_of(I.ctl), it, stage);
+        const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
+        // a later iteration's active tiles: this is the kernel that visits
+        // all of them, so it marks the scan candidates
+        if (stamp != 0 && cw.run) mark_candidates(I, it, marks, stamp);
+        update_tile(I, cw, it, stage);
     }, c1.run | c1.any_inadm | c1.mode);
 }
 
@@ -1012,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
 // pass's scan writes.
 __global__ void k_begin(const Inst* __restrict__ insts, int ninst, int stage, int32_t* __restrict__ scan_count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *scan_count = 0;
+    if (k == 0) *reinterpret_cast<unsigned long long*>(scan_count) = 0;  // both counts of list 0
     if (k >= ninst) return;
     int32_t* c = insts[k].ctl;
     if (!insts[k].active) {
@@ -1066,9 +1071,14 @@ __device__ __forceinline__ void publish_snapshot(int32_t* __restrict__ snap, int
 __global__ __launch_bounds__(256) void k_finish(int ninst, int stage, int32_t* __restrict__ ctl, int w,
                                                 int32_t* __restrict__ snap, uint32_t seq) {
     loop_decisions(ctl, ninst, stage);
-    if (threadIdx.x == 0) ctl[CTL_N * ninst + (1 - w)] = 0;
+    if (threadIdx.x == 0) {
+        int32_t* cnt = ctl + CTL_N * ninst;
+        snap[CTL_N * ninst + w] = cnt[2 * w];
+        snap[CTL_N * ninst + 3] = cnt[2 * w + 1];
+        *reinterpret_cast<unsigned long long*>(cnt + 2 * (1 - w)) = 0;
+    }
     __syncthreads();
-    for (int t = threadIdx.x; t < CTL_N * ninst + 2; t += blockDim.x) snap[t] = ctl[t];
+    for (int t = threadIdx.x; t < CTL_N * ninst; t += blockDim.x) snap[t] = ctl[t];
     publish_snapshot(snap, ninst, seq);
 }
 
@@ -1084,19 +1094,21 @@ constexpr int kScanTiles = 4;
 // With `snap` set, one more block ends the MOOD iteration as k_finish does
 // (the decisions read only flags earlier kernels set, and no scan or sync
 // block reads a word they write): the decisions, the control words copied
-// to `snap`, and, in snap's count slot w, the length of the list this
-// iteration consumed (list 1 - w, complete since the previous scan), whose
+// to `snap`, and, in snap's count slot w and word 3, the lengths of the
+// lists this iteration consumed (list 1 - w, complete since the previous
+// scan, and its face list), whose
 // count it then clears for the next scan; the sequence word `seq` last.
 template <bool ONE>
 __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
                                                                const Item* __restrict__ items, int npad,
-                                                               Item* __restrict__ act, int32_t* __restrict__ act_count,
+                                                               Item* __restrict__ act, Item* __restrict__ actf,
+                                                               int32_t* __restrict__ act_count,
                                                                const uint32_t* __restrict__ marks, uint32_t stamp,
                                                                int ninst, int stage, int32_t* __restrict__ ctl,
                                                                int wlist, int32_t* __restrict__ snap, uint32_t seq,
                                                                int sync_gx) {
-    __shared__ int hit[kScanTiles];
-    __shared__ int base;
+    __shared__ int hit[kScanTiles], hitf[kScanTiles];
+    __shared__ int base, basef;
     const int nsync = sync_gx * ninst, nfin = snap != nullptr ? 1 : 0;
     if ((int)blockIdx.x < nsync) {
         // the periodic MOOD sync, sync_gx blocks per solve
@@ -1109,9 +1121,10 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         loop_decisions(ctl, ninst, stage);
         __syncthreads();
         if (threadIdx.x == 0) {
-            int32_t* cnt = ctl + CTL_N * ninst + (1 - wlist);
-            snap[CTL_N * ninst + wlist] = *cnt;
-            *cnt = 0;
+            int32_t* cnt = ctl + CTL_N * ninst + 2 * (1 - wlist);
+            snap[CTL_N * ninst + wlist] = cnt[0];
+            snap[CTL_N * ninst + 3] = cnt[1];
+            *reinterpret_cast<unsigned long long*>(cnt) = 0;
         }
         for (int t = threadIdx.x; t < CTL_N * ninst; t += blockDim.x) snap[t] = ctl[t];
         publish_snapshot(snap, ninst, seq);
@@ -1125,13 +1138,13 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
     const int tc = t < npad ? t : (npad > 0 ? npad - 1 : 0);  // the list has room for one entry at least
     const Item it = ONE ? Item{0, (int16_t)(tc % one.ntx), (int16_t)(tc / one.ntx)} : items[tc];
     const Inst& I = inst_at<ONE>(insts, one, it.inst);
-    bool any = false;
+    bool any = false, anyf = false;
     if (t < npad && (stamp == 0 || marks[I.tile_off + it.ty * I.ntx + it.tx] == stamp)) {
         const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
         constexpr int W = kCellT + 4, NR = (W * W + 63) / 64;
         const size_t fp = (size_t)I.mx * I.my;
         // every cell of the footprint read at once (clamped, masked)
-        uint8_t f[NR];
+        uint8_t f[NR], fi[NR];
         bool in[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) {
@@ -1139,10 +1152,14 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
             in[r] = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
             const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
             const size_t idx = (size_t)cj * I.mx + ci;
-            f[r] = I.rc[idx] | I.rcF[idx] | inadm_cell(I, idx, fp);
+            fi[r] = inadm_cell(I, idx, fp);
+            f[r] = I.rc[idx] | I.rcF[idx] | fi[r];
         }
 #pragma unroll
-        for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
+        for (int r = 0; r < NR; r++) {
+            any |= in[r] && f[r] != 0;
+            anyf |= in[r] && fi[r] != 0;
+        }
         // The sync of this launch may not have reached the footprint's
         // guard / image bands yet: a band cell's rc and rcF are read as the
         // OR over its sync unit, which is their value once synced.
@@ -1162,32 +1179,49 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         }
     }
     const bool hw = __ballot(any) != 0;
+    // The face list.  Before an inner-loop round (CTL_LOWER), the face
+    // kernel has work only on faces with a side to re-reconstruct: a cell
+    // of the cross of an inadmissible cell, so the tile's footprint holds
+    // an inadmissible cell (no kernel of this launch writes those flags).
+    // Otherwise it is the active list.
+    const bool lower = I.ctl[CTL_LOWER] != 0;
+    const bool hf = lower ? __ballot(anyf) != 0 : hw;
     // The lowering of the next (inner-loop) iteration, on the tiles it will
     // visit.  Every cell it changes lies within two cells of an inadmissible
     // cell, so its tile is listed whatever this scan reads.  Other waves may
     // read flags it has just raised and list a tile the plain order would
     // not; such a tile holds no flag of its own and its kernels change
     // nothing.
-    if (hw && I.ctl[CTL_LOWER]) {
+    if (hw && lower) {
         const int ci0 = it.tx * kCellT, cj0 = it.ty * kCellT;
         for (int c = lane; c < kCellT * kCellT; c += 64) {
             const int i = ci0 + c % kCellT, j = cj0 + c / kCellT;
             if (i < I.mx && j < I.my) lower_cell(I, i, j);
         }
     }
-    if (lane == 0) hit[w] = hw ? 1 : 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int n = 0;
-#pragma unroll
-        for (int k = 0; k < kScanTiles; k++) n += hit[k];
-        base = n ? atomicAdd(act_count, n) : 0;
+    if (lane == 0) {
+        hit[w] = hw ? 1 : 0;
+        hitf[w] = hf ? 1 : 0;
     }
     __syncthreads();
-    if (lane == 0 && hw) {
-        int r = 0;
-        for (int k = 0; k < w; k++) r += hit[k];
-        act[base + r] = it;
+    if (threadIdx.x == 0) {
+        // both lists' counts in one 64-bit atomic (each below 2^31)
+        unsigned long long n = 0;
+#pragma unroll
+        for (int k = 0; k < kScanTiles; k++) n += (unsigned long long)hit[k] | (unsigned long long)hitf[k] << 32;
+        const unsigned long long b = n ? atomicAdd(reinterpret_cast<unsigned long long*>(act_count), n) : 0;
+        base = (int)(uint32_t)b;
+        basef = (int)(uint32_t)(b >> 32);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int r = 0, rf = 0;
+        for (int k = 0; k < w; k++) {
+            r += hit[k];
+            rf += hitf[k];
+        }
+        if (hw) act[base + r] = it;
+        if (hf) actf[basef + rf] = it;
     }
 }
 
@@ -1413,21 +1447,20 @@ void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int3
     const int nscan = L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1;
     const int sync_gx = L.sync_gx > 0 ? L.sync_gx : 0;
     HMHD_LAUNCH1(k_tile_scan, g1(sync_gx * L.ninst + (snap ? 1 : 0) + nscan), dim3(64 * kScanTiles), L.pad, L.npad,
-                 L.act[w],
-                 L.act_count + w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, seq, sync_gx);
+                 L.act[w], L.actf[w], L.act_count + 2 * w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, seq,
+                 sync_gx);
 }
 
-void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, uint32_t stamp,
-                 hipStream_t st) {
+void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
     if (grid <= 0) return;
     const dim3 g = g1(grid);
 #define HMHD_FACE(RR, ONE)                                                                                   \
     if (full)                                                                                                \
         hipLaunchKernelGGL((k_face<RR, true, ONE>), g, dim3(face_threads(RR, true)), 0, st, L.insts, L.inst1, \
-                           tl, stage, L.marks, stamp);                                                       \
+                           tl, stage);                                                                       \
     else                                                                                                     \
         hipLaunchKernelGGL((k_face<RR, false, ONE>), g, dim3(face_threads(RR, false)), 0, st, L.insts,        \
-                           L.inst1, tl, stage, L.marks, stamp);
+                           L.inst1, tl, stage);
 #define HMHD_FACE_R(ONE)   \
     if (R == 1) {          \
         HMHD_FACE(1, ONE)  \
@@ -1445,9 +1478,9 @@ void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, in
 #undef HMHD_FACE
 }
 
-void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st) {
+void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st) {
     if (grid <= 0) return;
-    HMHD_LAUNCH1(k_update, g1(grid), dim3(256), tl, stage);
+    HMHD_LAUNCH1(k_update, g1(grid), dim3(256), tl, stage, L.marks, stamp);
 }
 
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st) {

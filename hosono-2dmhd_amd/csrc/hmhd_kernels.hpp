@@ -19,16 +19,15 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t
 // `w`: which of the two active lists the scan of this iteration writes
 // stamp 0 scans every tile; otherwise only tiles marked with `stamp`.
 // With `snap` set the scan also ends the iteration as launch_finish does,
-// except that snap's count slot w gets the length of the list the
-// iteration consumed (list 1 - w)
+// except that snap's count slot w (and word 3) get the lengths of the lists
+// the iteration consumed (list 1 - w and its face list)
 // The periodic MOOD sync runs in the same launch (blocks ahead of the scan).
 void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
                       hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
-// later passes also mark the iteration's scan candidates with `stamp`
-void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, uint32_t stamp,
-                 hipStream_t st);
-void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, hipStream_t st);
+void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
+// stamp != 0 (a later iteration): also marks the iteration's scan candidates
+void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st);
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st);
 void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st);
 void launch_set_dt(Inst* insts, const double* dt, int ninst, hipStream_t st);

@@ -290,7 +290,8 @@ int Engine::commit() {
     if ((rc = ensure_stream())) return rc;
     if ((rc = sync())) return rc;
     const size_t n = inst_.size();
-    const size_t nctl = (size_t)CTL_N * n + 4;
+    // CTL_N words per solve, the four list counts, npad, one spare
+    const size_t nctl = (size_t)CTL_N * n + 6;
     if ((rc = grow(&d_insts_, cap_insts_, n, false))) return rc;
     if ((rc = grow(&d_ctl_, cap_ctl_, nctl, false))) return rc;
     // snapshots are polled while the stream runs: coherent (fine-grained) memory
@@ -351,7 +352,7 @@ int Engine::commit() {
     };
     if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
     if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
-    if ((rc = grow(&d_act_, cap_act_, 2 * (pad.size() + 1), false))) return rc;
+    if ((rc = grow(&d_act_, cap_act_, 4 * (pad.size() + 1), false))) return rc;
     if ((rc = grow(&d_marks_, cap_marks_, ntiles + 1, false))) return rc;
     if ((rc = hip_check(hipMemsetAsync(d_marks_, 0, sizeof(uint32_t) * (ntiles + 1), st_), "hipMemsetAsync(marks)")))
         return rc;
@@ -370,10 +371,12 @@ int Engine::commit() {
     L.ncell = (int)cell.size();
     L.act[0] = d_act_;
     L.act[1] = d_act_ + pad.size() + 1;
+    L.actf[0] = d_act_ + 2 * (pad.size() + 1);
+    L.actf[1] = d_act_ + 3 * (pad.size() + 1);
     L.act_count = d_ctl_ + (size_t)CTL_N * n;
-    L.npad_dev = d_ctl_ + (size_t)CTL_N * n + 2;
+    L.npad_dev = d_ctl_ + (size_t)CTL_N * n + 4;
     npad_val_ = (int32_t)pad.size();
-    if ((rc = hip_check(hipMemcpyAsync(d_ctl_ + (size_t)CTL_N * n + 2, &npad_val_, sizeof npad_val_,
+    if ((rc = hip_check(hipMemcpyAsync(d_ctl_ + (size_t)CTL_N * n + 4, &npad_val_, sizeof npad_val_,
                                        hipMemcpyHostToDevice, st_),
                         "hipMemcpyAsync(npad)")))
         return rc;
@@ -505,9 +508,9 @@ int Engine::read_snapshot(int w, bool visited) {
 }
 
 // templated kernels run once per GP radius present (tiles of other radii return)
-void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage, uint32_t stamp) {
+void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
     for (int R = 1; R <= kMaxR; R++)
-        if (has_R_[R]) launch_face(L, tl, grid, R, full, stage, stamp, st_);
+        if (has_R_[R]) launch_face(L, tl, grid, R, full, stage, st_);
 }
 
 // MOOD iteration it >= 1 of stage s: the active tiles listed by iteration
@@ -515,15 +518,16 @@ void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage, uin
 // snapshot into host slot it & 1.  `grid` blocks stride over the list.
 int Engine::enqueue_iteration(int s, int it, int grid) {
     const int r = (it - 1) & 1, w = it & 1;
-    const TileList act{L.act[r], L.act_count + r, 0};
+    const TileList act{L.act[r], L.act_count + 2 * r, 0};
+    const TileList actf{L.actf[r], L.act_count + 2 * r + 1, 0};
     // candidate stamps are never 0 (0 asks the scan for every tile)
     if (++stamp_ == 0) stamp_ = 1;
     // (an inner-loop round's lowering ran in the previous scan launch)
     prof_begin(PF_FACE_FLUX);
-    launch_face_groups(act, grid, false, s, stamp_);
+    launch_face_groups(actf, grid, false, s);
     prof_end(PF_FACE_FLUX);
     prof_begin(PF_UPDATE);
-    launch_update(L, act, grid, s, st_);
+    launch_update(L, act, grid, s, stamp_, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, act, grid, st_);
@@ -546,10 +550,10 @@ int Engine::run_stage(int s) {
     launch_begin(L, s, st_);
     launch_stage_init(L, st_);
     prof_begin(PF_FACE_FULL);
-    launch_face_groups(all, L.npad, true, s, 0);
+    launch_face_groups(all, L.npad, true, s);
     prof_end(PF_FACE_FULL);
     prof_begin(PF_UPDATE);
-    launch_update(L, all, L.npad, s, st_);
+    launch_update(L, all, L.npad, s, 0, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, all, L.npad, st_);

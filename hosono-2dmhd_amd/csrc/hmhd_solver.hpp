@@ -100,7 +100,7 @@ class Engine {
     void prof_begin(int fam);
     void prof_end(int fam);
     void prof_collect(bool wait_all);
-    void launch_face_groups(TileList tl, int grid, bool full, int stage, uint32_t stamp);
+    void launch_face_groups(TileList tl, int grid, bool full, int stage);
     template <class T>
     int grow(T** p, size_t& cap, size_t need, bool host, unsigned host_flags = hipHostMallocDefault);
 
