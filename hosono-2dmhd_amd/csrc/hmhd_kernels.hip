@@ -501,11 +501,26 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     if (threadIdx.x < 2) lcnt[threadIdx.x] = 0;
     __syncthreads();
 
+    // With one face per thread and a U tile of at most one cell per thread,
+    // each thread's cell of the tile is loaded together with its face's
+    // classification values, and written to LDS only if the tile has faces
+    // to rebuild: the rebuild waits on no further round trip.
+    constexpr bool kPre = NT == 2 * NC && LP <= NT;
+    const double* __restrict__ Uin = ubuf(I, stage_bufs(I.rot, stage).in);
+    const size_t uplane = (size_t)I.nx * I.ny;
+    double pre[NCONS];
     bool any_bad = false;
     for (int f = threadIdx.x; f < 2 * NC; f += NT) {
         const int d = f / NC, c = f - d * NC;
         const int i = i0 + c % TX, j = j0 + c / TX;
         const bool dom = i >= s - 2 && i <= ex + 3 && j >= s - 2 && j <= ey + 3;
+        if constexpr (kPre) {
+            const int u = f < LP ? f : LP - 1, uy = u / LX, ux = u - uy * LX;
+            const int kxx = map_axis(i0 - H + ux, s, I.nx, I.bc), kyy = map_axis(j0 - H + uy, s, I.ny, I.bc);
+            const double* src = Uin + (size_t)kyy * I.nx + kxx;
+#pragma unroll
+            for (int v = 0; v < NCONS; v++) pre[v] = src[v * uplane];
+        }
         // every value the classification reads, loaded at once from clamped
         // indices (a cell off the range reads a valid cell and is dropped)
         const int ci = i < 1 ? 1 : (i >= mx ? mx - 1 : i), cj = j < 1 ? 1 : (j >= I.my ? I.my - 1 : j);
@@ -547,15 +562,20 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     const int kx = lcnt[0], ky = lcnt[1], xw = (4 * kx + 63) & ~63;
     if (kx + ky > 0) {
         // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
-        const double* __restrict__ U = ubuf(I, stage_bufs(I.rot, stage).in);
-        const size_t plane = (size_t)I.nx * I.ny;
-        const int oi = i0 - H, oj = j0 - H;
-        for (int c = threadIdx.x; c < LP; c += NT) {
-            const int cy = c / LX, cx = c - cy * LX;
-            const int kxx = map_axis(oi + cx, s, I.nx, I.bc), kyy = map_axis(oj + cy, s, I.ny, I.bc);
-            const double* src = U + (size_t)kyy * I.nx + kxx;
+        if constexpr (kPre) {
+            if (threadIdx.x < LP) {
 #pragma unroll
-            for (int v = 0; v < NCONS; v++) sU[v * LP + c] = src[v * plane];
+                for (int v = 0; v < NCONS; v++) sU[v * LP + threadIdx.x] = pre[v];
+            }
+        } else {
+            const int oi = i0 - H, oj = j0 - H;
+            for (int c = threadIdx.x; c < LP; c += NT) {
+                const int cy = c / LX, cx = c - cy * LX;
+                const int kxx = map_axis(oi + cx, s, I.nx, I.bc), kyy = map_axis(oj + cy, s, I.ny, I.bc);
+                const double* src = Uin + (size_t)kyy * I.nx + kxx;
+#pragma unroll
+                for (int v = 0; v < NCONS; v++) sU[v * LP + c] = src[v * uplane];
+            }
         }
     }
     __syncthreads();
