@@ -69,6 +69,9 @@ struct Inst {
     uint8_t* rec;
     uint8_t* faceRR;  // [4][my][mx]
     uint8_t* fluxRR;  // [2][my][mx] radius of the last flux of the x / y face
+    // per 16 x 16 tile ([ty][ntx]): where the inadmissible face sides the
+    // face kernel's last visit of the tile left lie (bad_bits)
+    uint32_t* tbad;
     int32_t* ctl;
     unsigned long long* cflmax;  // [2] bit patterns of the max speeds
 };

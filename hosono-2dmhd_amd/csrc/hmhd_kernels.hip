@@ -284,6 +284,17 @@ constexpr int face_lds_cells() {
     return (kFaceTX + 2 * R + 1) * (kFaceTY + 2 * R + 1);
 }
 
+// Where in a tile the cell of an inadmissible face side lies, for the scan
+// of the tile and of its eight neighbours: bit 0 for any, then bits for a
+// cell within two cells of the tile's west, east, south and north edge.
+// (rx, ry): the cell relative to the tile's origin; -1 for a cell of the
+// west / south neighbour, whose east / north side belongs to this tile's
+// face.
+__device__ __forceinline__ uint32_t bad_bits(int rx, int ry) {
+    return 1u | (rx <= 1 ? 2u : 0u) | (rx >= kCellT - 2 ? 4u : 0u) | (ry <= 1 ? 8u : 0u) |
+           (ry >= kCellT - 2 ? 16u : 0u);
+}
+
 template <int R>
 __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int stage, double* __restrict__ sU) {
     constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, true);
@@ -305,6 +316,8 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
     const bool recC = cellC, recW = cellW, recS = cellS, fx = flux_cell, fy = flux_cell;
     const bool do_x = recC || recW || fx, do_y = recC || recS || fy;
 
+    uint32_t* const tbad = I.tbad + it.ty * I.ntx + it.tx;
+    if (threadIdx.x == 0) *tbad = 0;  // before the barrier, so the later ORs land on it
     // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
     {
         const double* __restrict__ U = ubuf(I, stage_bufs(I.rot, stage).in);
@@ -326,7 +339,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
     const int cc = (ly + H) * LX + (lx + H);  // LDS offset of cell (i, j)
     const LdsAcc<LX, LP> aC{sU + cc}, aW{sU + cc - 1}, aS{sU + cc - LX};
     uint8_t* const bad = I.inadm;  // planes [face][my][mx]
-    bool any_bad = false;
+    uint32_t bits = 0;
 
     {
         constexpr int NQ = gp_nq(R);
@@ -346,7 +359,8 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             }
             if (recW) bad[FACE_E * fplane + idx - 1] = badL;
             if (recC) bad[FACE_W * fplane + idx] = badR;
-            any_bad |= (recW && badL) || (recC && badR);
+            if (recW && badL) bits |= bad_bits(lx - 1, ly);
+            if (recC && badR) bits |= bad_bits(lx, ly);
             if (fx) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
@@ -370,7 +384,8 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             }
             if (recS) bad[FACE_N * fplane + idx - mx] = badL;
             if (recC) bad[FACE_S * fplane + idx] = badR;
-            any_bad |= (recS && badL) || (recC && badR);
+            if (recS && badL) bits |= bad_bits(lx, ly - 1);
+            if (recC && badR) bits |= bad_bits(lx, ly);
             if (fy) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
@@ -379,7 +394,10 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             }
         }
     }
-    if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
+    if (bits) {
+        I.ctl[CTL_ANY_INADM] = 1;
+        atomicOr(tbad, bits);
+    }
 }
 
 // One face of a later MOOD iteration on the four lanes of a quad, lane q
@@ -394,7 +412,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
 template <int R, int DIR, class A>
 __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set, const A& aL, const A& aR, size_t idx,
                                           size_t nb, bool recL, bool recR, bool flux, int rr_now, int storedL,
-                                          int storedR, int q, bool& any_bad) {
+                                          int storedR, int q, int lx, int ly, uint32_t& bits) {
     const size_t fplane = (size_t)I.mx * I.my;
     constexpr int fL = DIR == 0 ? FACE_E : FACE_N, fR = DIR == 0 ? FACE_W : FACE_S;
     const int rrL = recL ? rr_now : storedL;
@@ -443,7 +461,8 @@ __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set
         I.faceRR[fR * fplane + idx] = (uint8_t)rr_now;
         I.inadm[fR * fplane + idx] = (uint8_t)badR;
     }
-    any_bad |= (recL && badL) || (recR && badR);
+    if (recL && badL) bits |= bad_bits(DIR == 0 ? lx - 1 : lx, DIR == 0 ? ly : ly - 1);
+    if (recR && badR) bits |= bad_bits(lx, ly);
     if (flux) {
 #pragma unroll
         for (int v = 0; v < NCONS; v++) I.FR[(DIR * NCONS + v) * fplane + idx] = acc[v];
@@ -498,7 +517,9 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     const int i0 = it.tx * TX, j0 = it.ty * TY;
     const size_t fplane = (size_t)mx * I.my;
     const uint8_t* flag = cw.mode == MODE_NEWPASS ? I.rc : I.rec;
+    uint32_t* const tbad = I.tbad + it.ty * I.ntx + it.tx;
     if (threadIdx.x < 2) lcnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *tbad = 0;  // before the barrier, so the later ORs land on it
     __syncthreads();
 
     // With one face per thread and a U tile of at most one cell per thread,
@@ -509,7 +530,7 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     const double* __restrict__ Uin = ubuf(I, stage_bufs(I.rot, stage).in);
     const size_t uplane = (size_t)I.nx * I.ny;
     double pre[NCONS];
-    bool any_bad = false;
+    uint32_t bits = 0;
     for (int f = threadIdx.x; f < 2 * NC; f += NT) {
         const int d = f / NC, c = f - d * NC;
         const int i = i0 + c % TX, j = j0 + c / TX;
@@ -551,7 +572,8 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const bool keepF = !fl || rrF == rr_now;
         if (keepL && keepR && keepF) {
             if (fl) I.FLB[d * fplane + idx] = raw;
-            any_bad |= (recL && badL) || (recR && badR);
+            if (recL && badL) bits |= bad_bits(d == 0 ? c % TX - 1 : c % TX, d == 0 ? c / TX : c / TX - 1);
+            if (recR && badR) bits |= bad_bits(c % TX, c / TX);
         } else {
             const int slot = atomicAdd(&lcnt[d], 1);
             lst[d * NC + slot] = (uint32_t)c | (recL ? 0x100u : 0u) | (recR ? 0x200u : 0u) | (fl ? 0x400u : 0u) |
@@ -598,12 +620,15 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const LdsAcc<LX, LP> aR{sU + cc};
         if (isx)
             face_quad<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
-                            q, any_bad);
+                            q, lx, ly, bits);
         else
             face_quad<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
-                            rrR, q, any_bad);
+                            rrR, q, lx, ly, bits);
     }
-    if (any_bad) I.ctl[CTL_ANY_INADM] = 1;
+    if (bits) {
+        I.ctl[CTL_ANY_INADM] = 1;
+        atomicOr(tbad, bits);
+    }
 }
 
 template <int R, bool FULL, bool ONE>
@@ -1162,24 +1187,31 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
     if (t < npad && (stamp == 0 || marks[I.tile_off + it.ty * I.ntx + it.tx] == stamp)) {
         const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
         constexpr int W = kCellT + 4, NR = (W * W + 63) / 64;
-        const size_t fp = (size_t)I.mx * I.my;
-        // every cell of the footprint read at once (clamped, masked)
-        uint8_t f[NR], fi[NR];
+        // rc and rcF of every cell of the footprint, read at once (clamped,
+        // masked), and, in place of the inadmissible sides, the bad_bits of
+        // the 3 x 3 tiles around: every inadmissible side of a footprint
+        // cell was written by one of them
+        uint8_t f[NR];
         bool in[NR];
+        const int nty = (I.my + kCellT - 1) / kCellT, ta = it.tx + lane % 3 - 1, tb = it.ty + lane / 3 - 1;
+        const bool tin = lane < 9 && ta >= 0 && tb >= 0 && ta < I.ntx && tb < nty;
+        const uint32_t tf = I.tbad[(tin ? tb : it.ty) * I.ntx + (tin ? ta : it.tx)];
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int c = lane + r * 64, i = x0 + c % W, j = y0 + c / W;
             in[r] = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
             const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
             const size_t idx = (size_t)cj * I.mx + ci;
-            fi[r] = inadm_cell(I, idx, fp);
-            f[r] = I.rc[idx] | I.rcF[idx] | fi[r];
+            f[r] = I.rc[idx] | I.rcF[idx];
         }
+        // a neighbour's sides count when they lie within two cells of this
+        // tile: the edge (or corner) of the neighbour facing it
+        const int dx = lane % 3 - 1, dy = lane / 3 - 1;
+        const uint32_t need = 1u | (dx < 0 ? 4u : 0u) | (dx > 0 ? 2u : 0u) | (dy < 0 ? 16u : 0u) | (dy > 0 ? 8u : 0u);
+        anyf = tin && (tf & need) == need;
+        any = anyf;
 #pragma unroll
-        for (int r = 0; r < NR; r++) {
-            any |= in[r] && f[r] != 0;
-            anyf |= in[r] && fi[r] != 0;
-        }
+        for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
         // The sync of this launch may not have reached the footprint's
         // guard / image bands yet: a band cell's rc and rcF are read as the
         // OR over its sync unit, which is their value once synced.
@@ -1202,17 +1234,18 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
     // The face list.  Before an inner-loop round (CTL_LOWER), the face
     // kernel has work only on faces with a side to re-reconstruct: a cell
     // of the cross of an inadmissible cell, so the tile's footprint holds
-    // an inadmissible cell (no kernel of this launch writes those flags).
-    // Otherwise it is the active list.
+    // an inadmissible cell, which the bad_bits show (no kernel of this
+    // launch writes those; where they over-report, a tile listed without
+    // work changes nothing).  Otherwise it is the active list.
     const bool lower = I.ctl[CTL_LOWER] != 0;
     const bool hf = lower ? __ballot(anyf) != 0 : hw;
-    // The lowering of the next (inner-loop) iteration, on the tiles it will
-    // visit.  Every cell it changes lies within two cells of an inadmissible
-    // cell, so its tile is listed whatever this scan reads.  Other waves may
-    // read flags it has just raised and list a tile the plain order would
-    // not; such a tile holds no flag of its own and its kernels change
-    // nothing.
-    if (hw && lower) {
+    // The lowering of the next (inner-loop) iteration, on the tiles of the
+    // face list.  Every cell it changes lies within two cells of an
+    // inadmissible cell, so its tile is on that list whatever this scan
+    // reads.  Other waves may read flags it has just raised and list a tile
+    // the plain order would not; such a tile holds no flag of its own and
+    // its kernels change nothing.
+    if (hf && lower) {
         const int ci0 = it.tx * kCellT, cj0 = it.ty * kCellT;
         for (int c = lane; c < kCellT * kCellT; c += 64) {
             const int i = ci0 + c % kCellT, j = cj0 + c / kCellT;

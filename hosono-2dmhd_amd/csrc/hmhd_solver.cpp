@@ -83,7 +83,7 @@ Engine::~Engine() {
         gp_release(in.gpset);
     }
     void* dev[] = {d_insts_, d_ctl_,   d_cfl_,  d_dt_,    d_pad_,    d_cell_,
-                   d_act_,   d_marks_, d_part_, d_stage_, d_stage_i_};
+                   d_act_,   d_marks_, d_tbad_, d_part_, d_stage_, d_stage_i_};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     void* host[] = {h_ctl_, h_cfl_, h_dt_, h_scheme_};
@@ -342,6 +342,10 @@ int Engine::commit() {
         h.dev = d;
         table[k] = d;
     }
+    if ((rc = grow(&d_tbad_, cap_tbad_, ntiles + 1, false))) return rc;
+    if ((rc = hip_check(hipMemsetAsync(d_tbad_, 0, sizeof(uint32_t) * (ntiles + 1), st_), "hipMemsetAsync(tbad)")))
+        return rc;
+    for (size_t k = 0; k < n; k++) inst_[k].dev.tbad = table[k].tbad = d_tbad_ + table[k].tile_off;
     if (n && (rc = hip_check(hipMemcpy(d_insts_, table.data(), sizeof(Inst) * n, hipMemcpyHostToDevice),
                              "hipMemcpy(table)")))
         return rc;

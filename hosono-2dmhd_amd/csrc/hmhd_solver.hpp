@@ -139,6 +139,8 @@ class Engine {
     size_t cap_act_ = 0;
     uint32_t* d_marks_ = nullptr;  // scan candidates, one word per tile
     size_t cap_marks_ = 0;
+    uint32_t* d_tbad_ = nullptr;  // Inst::tbad of every solve
+    size_t cap_tbad_ = 0;
     uint32_t stamp_ = 0;           // last candidate stamp handed out
     unsigned long long* d_part_ = nullptr;
     size_t cap_part_ = 0;
