@@ -73,7 +73,6 @@ struct Inst {
     // face kernel's last visit of the tile left lie (bad_bits)
     uint32_t* tbad;
     int32_t* ctl;
-    unsigned long long* cflmax;  // [2] bit patterns of the max speeds
 };
 
 // one tile of one instance; the tile grid depends on the kernel family
@@ -100,7 +99,7 @@ struct TileList {
 
 struct LaunchSet {
     const Inst* insts;
-    int32_t* ctl;      // CTL_N words per instance, then the list counts (act_count), then npad
+    int32_t* ctl;      // CTL_N words per instance, the list counts (act_count), npad, the CFL block counter
     const Item* pad;   // every tile of the padded grids
     int npad;
     int ninst;

@@ -237,8 +237,11 @@ __device__ __forceinline__ void accumulate_flux(double* acc, const Phys& ph, con
 }
 
 // ------------------------------------------------------------------ //
-// stage set-up: scheme = R, rc = rcF = true everywhere
-// (src/timeStep.f90:79-81); inadm cleared; every face at radius R
+// stage set-up: rc = rcF = true everywhere (src/timeStep.f90:80-81);
+// inadm cleared; every face at radius R.  The scheme's reset to R
+// (src/timeStep.f90:79) is left to the first-pass k_update, the first
+// kernel to read it: the scheme keeps the last step's values until dt is
+// known, though cfl() enqueues the start of the next step's first pass.
 __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ insts, const Item* __restrict__ items) {
     const Item it = items[xcd_tile(blockIdx.x, gridDim.x)];
     const Inst& I = insts[it.inst];
@@ -246,7 +249,6 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
     if (i >= I.mx || j >= I.my) return;
     const size_t idx = (size_t)j * I.mx + i, plane = (size_t)I.mx * I.my;
     const uint8_t R = (uint8_t)I.R;
-    I.scheme[idx] = R;
     I.rc[idx] = 1;
     I.rcF[idx] = 1;
 #pragma unroll
@@ -1050,9 +1052,16 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
-        // a later iteration's active tiles: this is the kernel that visits
-        // all of them, so it marks the scan candidates
-        if (stamp != 0 && cw.run) mark_candidates(I, it, marks, stamp);
+        if (stamp != 0) {
+            // a later iteration's active tiles: this is the kernel that
+            // visits all of them, so it marks the scan candidates
+            if (cw.run) mark_candidates(I, it, marks, stamp);
+        } else {
+            // the first pass, every tile: the stage set-up's scheme = R
+            // (src/timeStep.f90:79), read first here
+            const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+            if (i < I.mx && j < I.my) I.scheme[(size_t)j * I.mx + i] = (uint8_t)I.R;
+        }
         update_tile(I, cw, it, stage);
     }, c1.run | c1.any_inadm | c1.mode);
 }
@@ -1290,12 +1299,13 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, const Item* __restrict__ items,
-                                             unsigned long long* __restrict__ part) {
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, const Inst one,
+                                             const Item* __restrict__ items, unsigned long long* __restrict__ part) {
     // partials are stored per tile, so each instance's lie at [cell_off, cell_off + cell_cnt)
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const Item it = items[t];
-    const Inst& I = insts[it.inst];
+    const Inst& I = inst_at<ONE>(insts, one, it.inst);
     const int ki = it.tx * kCellT + (threadIdx.x % kCellT), kj = it.ty * kCellT + (threadIdx.x / kCellT);
     unsigned long long bx = 0, by = 0;
     if (ki < I.nx && kj < I.ny) {
@@ -1358,9 +1368,14 @@ __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, con
     }
 }
 
-// second level: one block per instance over its block partials
+// second level: one block per instance over its block partials; the two
+// maxima of instance k go to out[2k], out[2k + 1] (pinned host memory),
+// and the last block to finish stores `seq` at *seqp, after every block's
+// maxima (`done` counts the blocks and is left at zero)
 __global__ __launch_bounds__(256) void k_cfl_reduce(const Inst* __restrict__ insts,
-                                                    const unsigned long long* __restrict__ part) {
+                                                    const unsigned long long* __restrict__ part,
+                                                    unsigned long long* __restrict__ out, uint32_t* __restrict__ done,
+                                                    int32_t* __restrict__ seqp, uint32_t seq) {
     const Inst& I = insts[blockIdx.x];
     unsigned long long bx = 0, by = 0;
     for (int b = I.cell_off + threadIdx.x; b < I.cell_off + I.cell_cnt; b += blockDim.x) {
@@ -1380,8 +1395,14 @@ __global__ __launch_bounds__(256) void k_cfl_reduce(const Inst* __restrict__ ins
             bx = wx[w] > bx ? wx[w] : bx;
             by = wy[w] > by ? wy[w] : by;
         }
-        I.cflmax[0] = bx;
-        I.cflmax[1] = by;
+        out[2 * blockIdx.x] = bx;
+        out[2 * blockIdx.x + 1] = by;
+        __threadfence_system();
+        if (atomicAdd(done, 1u) == gridDim.x - 1) {
+            *done = 0;
+            __threadfence_system();
+            __hip_atomic_store(seqp, (int32_t)seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -1541,9 +1562,15 @@ void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t 
 }
 #undef HMHD_LAUNCH1
 
-void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st) {
-    if (L.ncell > 0) hipLaunchKernelGGL(k_cfl, g1(L.ncell), dim3(256), 0, st, L.insts, L.cell, L.cflpart);
-    hipLaunchKernelGGL(k_cfl_reduce, dim3(ninst), dim3(256), 0, st, L.insts, L.cflpart);
+void launch_cfl(const LaunchSet& L, int ninst, unsigned long long* out, uint32_t* done, int32_t* seqp, uint32_t seq,
+                hipStream_t st) {
+    if (L.ncell > 0) {
+        if (L.one)
+            hipLaunchKernelGGL(k_cfl<true>, g1(L.ncell), dim3(256), 0, st, L.insts, L.inst1, L.cell, L.cflpart);
+        else
+            hipLaunchKernelGGL(k_cfl<false>, g1(L.ncell), dim3(256), 0, st, L.insts, L.inst1, L.cell, L.cflpart);
+    }
+    hipLaunchKernelGGL(k_cfl_reduce, dim3(ninst), dim3(256), 0, st, L.insts, L.cflpart, out, done, seqp, seq);
 }
 
 void launch_set_dt(Inst* insts, const double* dt, int ninst, hipStream_t st) {

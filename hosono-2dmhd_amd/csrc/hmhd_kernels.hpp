@@ -25,11 +25,16 @@ void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t
 void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
                       hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
+// full: the first pass (every face)
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
-// stamp != 0 (a later iteration): also marks the iteration's scan candidates
+// stamp != 0 (a later iteration): also marks the iteration's scan
+// candidates; stamp == 0 (the first pass): also resets the scheme to R
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st);
 void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st);
-void launch_cfl(const LaunchSet& L, int ninst, hipStream_t st);
+// CFL maxima of every solve into `out` (pinned host memory, two per solve);
+// the sequence word *seqp is set to `seq` once all are there
+void launch_cfl(const LaunchSet& L, int ninst, unsigned long long* out, uint32_t* done, int32_t* seqp, uint32_t seq,
+                hipStream_t st);
 void launch_set_dt(Inst* insts, const double* dt, int ninst, hipStream_t st);
 void launch_rotate(Inst* insts, int ninst, hipStream_t st);
 void launch_unpack(const Inst& I, const double* src, int nv, bool to_v0, int buf, hipStream_t st);

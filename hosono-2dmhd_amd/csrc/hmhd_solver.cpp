@@ -82,7 +82,7 @@ Engine::~Engine() {
         if (in.block) (void)hipFree(in.block);
         gp_release(in.gpset);
     }
-    void* dev[] = {d_insts_, d_ctl_,   d_cfl_,  d_dt_,    d_pad_,    d_cell_,
+    void* dev[] = {d_insts_, d_ctl_,   d_dt_,    d_pad_,    d_cell_,
                    d_act_,   d_marks_, d_tbad_, d_part_, d_stage_, d_stage_i_};
     for (void* p : dev)
         if (p) (void)hipFree(p);
@@ -205,6 +205,7 @@ int64_t Engine::add(const hmhd_config& c) {
     index_[h.id] = inst_.size();
     inst_.push_back(std::move(h));
     dirty_ = true;
+    pre_ok_ = false;
     return inst_.back().id;
 }
 
@@ -241,6 +242,7 @@ int Engine::remove(int64_t id) {
     index_.clear();
     for (size_t k = 0; k < inst_.size(); k++) index_[inst_[k].id] = k;
     dirty_ = true;
+    pre_ok_ = false;
     return HMHD_OK;
 }
 
@@ -253,6 +255,7 @@ int Engine::set_active(int64_t id, bool on) {
     if (h->active != on) {
         h->active = on;
         dirty_ = true;
+        pre_ok_ = false;
     }
     return HMHD_OK;
 }
@@ -290,7 +293,7 @@ int Engine::commit() {
     if ((rc = ensure_stream())) return rc;
     if ((rc = sync())) return rc;
     const size_t n = inst_.size();
-    // CTL_N words per solve, the four list counts, npad, one spare
+    // CTL_N words per solve, the four list counts, npad, the CFL block counter
     const size_t nctl = (size_t)CTL_N * n + 6;
     if ((rc = grow(&d_insts_, cap_insts_, n, false))) return rc;
     if ((rc = grow(&d_ctl_, cap_ctl_, nctl, false))) return rc;
@@ -299,8 +302,11 @@ int Engine::commit() {
     if ((rc = hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_hctl_), h_ctl_, 0),
                         "hipHostGetDevicePointer")))
         return rc;
-    if ((rc = grow(&d_cfl_, cap_cfl_, 2 * n, false))) return rc;
-    if ((rc = grow(&h_cfl_, cap_hcfl_, 2 * n, true))) return rc;
+    // polled while the stream runs, like the snapshots
+    if ((rc = grow(&h_cfl_, cap_hcfl_, 2 * n + 2, true, hipHostMallocCoherent | hipHostMallocMapped))) return rc;
+    if ((rc = hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_hcfl_), h_cfl_, 0),
+                        "hipHostGetDevicePointer")))
+        return rc;
     if ((rc = grow(&d_dt_, cap_dt_, n, false))) return rc;
     if ((rc = grow(&h_dt_, cap_hdt_, n, true))) return rc;
     if ((rc = hip_check(hipMemsetAsync(d_ctl_, 0, sizeof(int32_t) * nctl, st_), "hipMemsetAsync(ctl)"))) return rc;
@@ -316,7 +322,6 @@ int Engine::commit() {
         InstHost& h = inst_[k];
         Inst d = h.dev;
         d.ctl = d_ctl_ + CTL_N * k;
-        d.cflmax = d_cfl_ + 2 * k;
         d.active = h.active ? 1 : 0;
         d.v0 = h.v0_valid ? 1 : 0;
         d.cell_off = (int32_t)cell.size();
@@ -408,6 +413,7 @@ int Engine::set_gp(int64_t id, const double* pv, const double* qw) {
     std::memcpy(h->pv, pv, sizeof h->pv);
     std::memcpy(h->qw, qw, sizeof h->qw);
     dirty_ = true;
+    pre_ok_ = false;
     return HMHD_OK;
 }
 
@@ -541,21 +547,30 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     return hip_check(hipGetLastError(), "kernel launch");
 }
 
+// The start of stage s: the loop's control words, the flags' stage set-up
+// and the first pass's face kernel (every face at the top radius).
+// Nothing here reads dt or changes the scheme.
+void Engine::first_pass_faces(int s) {
+    const TileList all{L.pad, L.npad_dev, 1};
+    launch_begin(L, s, st_);
+    launch_stage_init(L, st_);
+    prof_begin(PF_FACE_FULL);
+    launch_face_groups(all, L.npad, true, s);
+    prof_end(PF_FACE_FULL);
+}
+
 // One RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active
 // solve.  The loop runs one iteration ahead of the host: iteration it + 1
 // is enqueued before the host has read whether iteration it left work.  A
 // speculative iteration after the last needed one is a no-op (k_finish has
 // ended every solve's loop, and every kernel then returns), so the results
 // and the pass counts are those of the plain loop.
-int Engine::run_stage(int s) {
+int Engine::run_stage(int s, bool pre) {
     const int n = (int)inst_.size();
     const TileList all{L.pad, L.npad_dev, 1};
     // iteration 0, the first pass: every tile, every face at the top radius
-    launch_begin(L, s, st_);
-    launch_stage_init(L, st_);
-    prof_begin(PF_FACE_FULL);
-    launch_face_groups(all, L.npad, true, s);
-    prof_end(PF_FACE_FULL);
+    // (its start already enqueued by cfl() when `pre`)
+    if (!pre) first_pass_faces(s);
     prof_begin(PF_UPDATE);
     launch_update(L, all, L.npad, s, 0, st_);
     prof_end(PF_UPDATE);
@@ -612,6 +627,9 @@ int Engine::run_stage(int s) {
 }
 
 int Engine::advance(const std::vector<double>& dt) {
+    // cfl()'s first-pass face work stands if nothing changed since
+    const bool pre = pre_ok_ && !dirty_;
+    pre_ok_ = false;
     int rc = commit();
     if (rc) return rc;
     const int n = (int)inst_.size();
@@ -621,19 +639,24 @@ int Engine::advance(const std::vector<double>& dt) {
         err = "advance: one dt per active solve expected";
         return HMHD_ERR_ARG;
     }
-    if ((rc = hip_check(hipMemcpyAsync(d_dt_, h_dt_, sizeof(double) * n, hipMemcpyHostToDevice, st_),
-                        "hipMemcpyAsync(dt)")))
-        return rc;
-    launch_set_dt(d_insts_, d_dt_, n, st_);
     if (L.one) {
-        // the single-solve kernels' copy of the record, as k_set_dt writes it
+        // the single-solve kernels take the record, dt/dx and dt/dy as
+        // k_set_dt computes them, from their arguments; the table's copy is
+        // not read
         L.inst1.dtdx = h_dt_[0] / L.inst1.dx;
         L.inst1.dtdy = h_dt_[0] / L.inst1.dy;
+    } else {
+        if ((rc = hip_check(hipMemcpyAsync(d_dt_, h_dt_, sizeof(double) * n, hipMemcpyHostToDevice, st_),
+                            "hipMemcpyAsync(dt)")))
+            return rc;
+        launch_set_dt(d_insts_, d_dt_, n, st_);
     }
     // U1 = FE(U); U2 = (3U + FE(U1))/4; U^{n+1} = (U + 2 FE(U2))/3
     for (int s = 0; s < 3; s++)
-        if ((rc = run_stage(s))) return rc;
-    launch_rotate(d_insts_, n, st_);
+        if ((rc = run_stage(s, s == 0 && pre))) return rc;
+    // (the single-solve kernels take rot and v0 from their arguments, and
+    // commit() rebuilds the table from the host records)
+    if (!L.one) launch_rotate(d_insts_, n, st_);
     if (L.one && L.inst1.active) {
         // as k_rotate
         L.inst1.rot = (L.inst1.rot + 1) % 3;
@@ -677,7 +700,8 @@ int Engine::upload(int64_t id, const double* U, const double* V) {
         h->upV.clear();
     }
     h->v0_valid = V != nullptr;
-    dirty_ = true;  // the device table carries the v0 flag
+    dirty_ = true;    // the device table carries the v0 flag
+    pre_ok_ = false;  // and cfl()'s first-pass face work read the old state
     return HMHD_OK;
 }
 
@@ -685,21 +709,46 @@ int Engine::cfl(std::vector<double>& dt_out) {
     int rc = commit();
     if (rc) return rc;
     const int n = (int)inst_.size();
+    if (++cfl_seq_ == 0) cfl_seq_ = 1;
+    volatile int32_t* seq = reinterpret_cast<volatile int32_t*>(h_cfl_ + 2 * n);
     prof_begin(PF_CFL);
-    launch_cfl(L, n, st_);
+    launch_cfl(L, n, d_hcfl_, reinterpret_cast<uint32_t*>(d_ctl_ + (size_t)CTL_N * n + 5),
+               reinterpret_cast<int32_t*>(d_hcfl_ + 2 * n), cfl_seq_, st_);
     prof_end(PF_CFL);
-    if ((rc = hip_check(hipMemcpyAsync(h_cfl_, d_cfl_, sizeof(unsigned long long) * 2 * n, hipMemcpyDeviceToHost,
-                                       st_),
-                        "hipMemcpyAsync(cfl)")))
-        return rc;
-    if ((rc = sync())) return rc;
-    prof_collect(true);
+    if ((rc = hip_check(hipGetLastError(), "kernel launch"))) return rc;
+    // The next advance's first pass starts with face work that does not
+    // depend on dt: enqueue it now, so it runs while dt makes its way
+    // through the host.  The state it reads is the one this CFL reads.
+    bool any = false;
+    for (const auto& h : inst_) any = any || h.active;
+    if (any && !pre_ok_) {
+        first_pass_faces(0);
+        if ((rc = hip_check(hipGetLastError(), "kernel launch"))) return rc;
+        pre_ok_ = true;
+    }
+    // the maxima are in once the sequence word is (stored last, behind a
+    // system-scope fence); an idle stream without it is an error
+    for (unsigned k = 1; *seq != (int32_t)cfl_seq_; k++) {
+        if (k % 1024 == 0) {
+            const hipError_t e = hipStreamQuery(st_);
+            if (e == hipSuccess && *seq != (int32_t)cfl_seq_) {
+                err = "CFL result missing";
+                return HMHD_ERR_DEVICE;
+            }
+            if (e != hipSuccess && e != hipErrorNotReady) return hip_check(e, "hipStreamQuery");
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    prof_collect(false);
     dt_out.clear();
     for (int k = 0; k < n; k++) {
         if (!inst_[k].active) continue;
         double xm, ym;
-        std::memcpy(&xm, &h_cfl_[2 * k], 8);
-        std::memcpy(&ym, &h_cfl_[2 * k + 1], 8);
+        const unsigned long long bx = reinterpret_cast<volatile unsigned long long*>(h_cfl_)[2 * k];
+        const unsigned long long by = reinterpret_cast<volatile unsigned long long*>(h_cfl_)[2 * k + 1];
+        std::memcpy(&xm, &bx, 8);
+        std::memcpy(&ym, &by, 8);
         const Inst& d = inst_[k].dev;
         // dt = cfl * min(dx/xmax, dy/ymax) (src/cfl.f90:50); a NaN speed
         // gives a NaN dt so the driver's validTimeStep aborts

@@ -96,7 +96,8 @@ class Engine {
     int snapshot(int s, int w);
     int read_snapshot(int w, bool visited);
     int enqueue_iteration(int s, int it, int grid);
-    int run_stage(int s);
+    int run_stage(int s, bool pre);
+    void first_pass_faces(int s);
     void prof_begin(int fam);
     void prof_end(int fam);
     void prof_collect(bool wait_all);
@@ -123,10 +124,14 @@ class Engine {
     size_t snap_stride() const { return (size_t)CTL_N * inst_.size() + 4; }
     uint32_t next_seq(int w);
     uint32_t snap_seq_ = 0, snap_want_[2] = {0, 0};
-    unsigned long long* d_cfl_ = nullptr;  // two maxima per solve
-    size_t cap_cfl_ = 0;
+    // CFL maxima, two per solve, then a sequence word: pinned, written by the device
     unsigned long long* h_cfl_ = nullptr;
+    unsigned long long* d_hcfl_ = nullptr;  // its device address
     size_t cap_hcfl_ = 0;
+    uint32_t cfl_seq_ = 0;
+    // the next advance's first-pass face work (stage 0) is already enqueued,
+    // by cfl(), for the state and membership as they are
+    bool pre_ok_ = false;
     double* d_dt_ = nullptr;
     size_t cap_dt_ = 0;
     double* h_dt_ = nullptr;
