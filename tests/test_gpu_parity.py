@@ -121,6 +121,47 @@ def test_late_time_ot_with_heavy_mood_activity():
     assert total_passes > 15, "expected MOOD passes beyond the first in most stages"
 
 
+def test_calls_between_cfl_and_advance():
+    """cfl_computedt starts the next step's first pass ahead of dt.  A
+    download in between still returns the state and scheme of the last
+    step, a second CFL the same dt, and the step that follows is the plain
+    one."""
+    cfg, g, orc, gpu = _pair("ot")
+    for n in range(3):
+        _step_both(orc, gpu)
+    Uo, Vo = orc.get_state()
+    last_scheme = orc.stage_schemes()[2]
+    d = gpu.cfl_computedt()
+    Ug, Vg, Sg = gpu.download()
+    assert np.array_equal(Uo, Ug) and np.array_equal(Vo, Vg)
+    assert np.array_equal(last_scheme, Sg), "scheme changed by cfl_computedt"
+    assert gpu.cfl_computedt() == d == orc.cfl_dt()
+    for n in range(2):
+        _step_both(orc, gpu)
+        _compare_state(orc, gpu, f"after cfl/download/cfl, step {n + 1}")
+
+
+def test_upload_and_gp_change_after_cfl():
+    """State or GP tables changed between cfl_computedt and the step: the
+    first-pass work cfl_computedt started is dropped and redone."""
+    cfg, g, orc, gpu = _pair("blast")
+    for n in range(2):
+        _step_both(orc, gpu)
+    _, _, U0, V0 = cases.make("blast")  # back to the initial state
+    gpu.cfl_computedt()
+    orc.set_state(U0, V0)
+    gpu.upload(U0, V0)
+    for n in range(2):
+        _step_both(orc, gpu)
+        _compare_state(orc, gpu, f"after upload, step {n + 1}")
+    pv, qw, _ = gpu.gp_tables()
+    gpu.cfl_computedt()
+    gpu.set_gp_tables(pv, qw)
+    for n in range(2):
+        _step_both(orc, gpu)
+        _compare_state(orc, gpu, f"after set_gp, step {n + 1}")
+
+
 def test_errors_are_reported():
     with pytest.raises(rt.HmhdError):
         rt.Solver(rt.Config(64, 64, R=4))
