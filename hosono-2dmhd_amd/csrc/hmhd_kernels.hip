@@ -218,24 +218,6 @@ __device__ __forceinline__ StageBufs stage_bufs(int rot, int stage) {
 // state buffer k of a solve, selected without indexing the pointer array
 __device__ __forceinline__ double* ubuf(const Inst& I, int k) { return k == 0 ? I.U[0] : (k == 1 ? I.U[1] : I.U[2]); }
 
-template <int DIR>
-__device__ __forceinline__ void accumulate_flux(double* acc, const Phys& ph, const double* uL, const double* uR,
-                                                double w, bool first, bool fog) {
-    double F[NCONS];
-    hllc<DIR>(ph, uL, uR, F);
-    if (fog) {
-#pragma unroll
-        for (int v = 0; v < NCONS; v++) acc[v] = F[v];
-    } else {
-        if (first) {
-#pragma unroll
-            for (int v = 0; v < NCONS; v++) acc[v] = 0.0;
-        }
-#pragma unroll
-        for (int v = 0; v < NCONS; v++) acc[v] = acc[v] + w * F[v];
-    }
-}
-
 // ------------------------------------------------------------------ //
 // stage set-up: rc = rcF = true everywhere (src/timeStep.f90:80-81);
 // inadm cleared; every face at radius R.  The scheme's reset to R
@@ -298,7 +280,8 @@ __device__ __forceinline__ uint32_t bad_bits(int rx, int ry) {
 }
 
 template <int R>
-__device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int stage, double* __restrict__ sU) {
+__device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int stage, double* __restrict__ sU,
+                                               double* __restrict__ sAcc) {
     constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, true);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
@@ -345,10 +328,13 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
 
     {
         constexpr int NQ = gp_nq(R);
+        // The flux sums over the Gauss points are kept in LDS (sAcc, a
+        // column per thread, stride NT), not in registers: the kernel is at
+        // its register limit, and the 16 registers spare it spills.
         // x-face: east state of (i-1, j) | west state of (i, j)
         if (do_x) {
             bool badL = false, badR = false;
-            double acc[NCONS];
+            double* const acc = sAcc + threadIdx.x;
 #pragma unroll 1
             for (int q = 0; q < NQ; q++) {
                 double uL[NCONS], uR[NCONS];
@@ -357,7 +343,13 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
                 gp_point<R>(aC, pv_ptr(set, R, FACE_W, q), uR);
                 badL |= !admissible(uL[DENS], cons2pres(ph, uL));
                 badR |= !admissible(uR[DENS], cons2pres(ph, uR));
-                if (fx) accumulate_flux<0>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
+                if (fx) {
+                    double F[NCONS];
+                    hllc<0>(ph, uL, uR, F);
+                    const double wq = c_qw[set][R][q];
+#pragma unroll
+                    for (int v = 0; v < NCONS; v++) acc[v * NT] = (q == 0 ? 0.0 : acc[v * NT]) + wq * F[v];
+                }
             }
             if (recW) bad[FACE_E * fplane + idx - 1] = badL;
             if (recC) bad[FACE_W * fplane + idx] = badR;
@@ -365,15 +357,15 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             if (recC && badR) bits |= bad_bits(lx, ly);
             if (fx) {
 #pragma unroll
-                for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v];
-                I.FLB[idx] = acc[MAGY];
+                for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v * NT];
+                I.FLB[idx] = acc[MAGY * NT];
                 I.fluxRR[idx] = (uint8_t)R;
             }
         }
         // y-face: north state of (i, j-1) | south state of (i, j)
         if (do_y) {
             bool badL = false, badR = false;
-            double acc[NCONS];
+            double* const acc = sAcc + threadIdx.x;
 #pragma unroll 1
             for (int q = 0; q < NQ; q++) {
                 double uL[NCONS], uR[NCONS];
@@ -382,7 +374,13 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
                 gp_point<R>(aC, pv_ptr(set, R, FACE_S, q), uR);
                 badL |= !admissible(uL[DENS], cons2pres(ph, uL));
                 badR |= !admissible(uR[DENS], cons2pres(ph, uR));
-                if (fy) accumulate_flux<1>(acc, ph, uL, uR, c_qw[set][R][q], q == 0, false);
+                if (fy) {
+                    double F[NCONS];
+                    hllc<1>(ph, uL, uR, F);
+                    const double wq = c_qw[set][R][q];
+#pragma unroll
+                    for (int v = 0; v < NCONS; v++) acc[v * NT] = (q == 0 ? 0.0 : acc[v * NT]) + wq * F[v];
+                }
             }
             if (recS) bad[FACE_N * fplane + idx - mx] = badL;
             if (recC) bad[FACE_S * fplane + idx] = badR;
@@ -390,8 +388,8 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             if (recC && badR) bits |= bad_bits(lx, ly);
             if (fy) {
 #pragma unroll
-                for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v];
-                I.FLB[fplane + idx] = acc[MAGX];
+                for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v * NT];
+                I.FLB[fplane + idx] = acc[MAGX * NT];
                 I.fluxRR[fplane + idx] = (uint8_t)R;
             }
         }
@@ -637,6 +635,7 @@ template <int R, bool FULL, bool ONE>
 __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
     const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
+    __shared__ double sAcc[FULL ? NCONS * face_threads(R, true) : 1];
     __shared__ uint32_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
     __shared__ int lcnt[2];
     const Ctl c1 = ONE && !FULL ? ctl_of(one.ctl) : Ctl{};
@@ -646,7 +645,7 @@ __global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES
         later = true;
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         if (FULL)
-            face_tile_full<R>(I, it, stage, sU);
+            face_tile_full<R>(I, it, stage, sU, sAcc);
         else
             face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
     }, c1.run | c1.mode);
