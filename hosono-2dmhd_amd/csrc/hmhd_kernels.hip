@@ -278,6 +278,19 @@ __device__ __forceinline__ uint32_t bad_bits(int rx, int ry) {
     return 1u | (rx <= 1 ? 2u : 0u) | (rx >= kCellT - 2 ? 4u : 0u) | (ry <= 1 ? 8u : 0u) |
            (ry >= kCellT - 2 ? 16u : 0u);
 }
+// The end of a face tile: the bad_bits of the wave's active lanes go into
+// the tile's word with one atomic, and ANY_INADM is raised, by the first
+// active lane (a tile with many inadmissible sides would otherwise queue
+// hundreds of atomics on one word).
+__device__ __forceinline__ void report_bad(const Inst& I, uint32_t* tbad, uint32_t bits) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) w |= __ballot((bits >> k) & 1u) != 0 ? 1u << k : 0u;
+    if (w && (int)(threadIdx.x & 63u) == __ffsll(__ballot(1)) - 1) {
+        I.ctl[CTL_ANY_INADM] = 1;
+        atomicOr(tbad, w);
+    }
+}
 
 template <int R>
 __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int stage, double* __restrict__ sU,
@@ -394,10 +407,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             }
         }
     }
-    if (bits) {
-        I.ctl[CTL_ANY_INADM] = 1;
-        atomicOr(tbad, bits);
-    }
+    report_bad(I, tbad, bits);
 }
 
 // One face of a later MOOD iteration on the four lanes of a quad, lane q
@@ -625,10 +635,7 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
             face_quad<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
                             rrR, q, lx, ly, bits);
     }
-    if (bits) {
-        I.ctl[CTL_ANY_INADM] = 1;
-        atomicOr(tbad, bits);
-    }
+    report_bad(I, tbad, bits);
 }
 
 template <int R, bool FULL, bool ONE>
