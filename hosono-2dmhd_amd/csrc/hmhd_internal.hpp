@@ -24,7 +24,7 @@ namespace hmhd {
 
 enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
 enum : int { NCONS = 8 };
-// ctl words: flags the kernels raise, the MOOD-loop state k_begin and
+// ctl words: flags the kernels raise, the MOOD-loop state k_stage_init and
 // k_finish keep on the device, and per-stage pass / inner-iteration counters
 enum : int {
     CTL_ANY_INADM = 0,  // an inadmissible reconstructed state was found

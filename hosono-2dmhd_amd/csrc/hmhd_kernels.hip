@@ -224,7 +224,28 @@ __device__ __forceinline__ double* ubuf(const Inst& I, int k) { return k == 0 ? 
 // (src/timeStep.f90:79) is left to the first-pass k_update, the first
 // kernel to read it: the scheme keeps the last step's values until dt is
 // known, though cfl() enqueues the start of the next step's first pass.
-__global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ insts, const Item* __restrict__ items) {
+// Block 0 also starts the stage's MOOD loop, one thread per instance
+// (inactive instances stay out of the loop), and empties the active list
+// the first pass's scan writes; no other block reads the control words.
+__global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ insts, const Item* __restrict__ items,
+                                                    int npad, int ninst, int stage, int32_t* __restrict__ scan_count) {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) *reinterpret_cast<unsigned long long*>(scan_count) = 0;  // both counts of list 0
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+            int32_t* c = insts[k].ctl;
+            if (!insts[k].active) {
+                c[CTL_RUN] = 0;
+                continue;
+            }
+            c[CTL_ANY_INADM] = 0;
+            c[CTL_ANY_LOWER] = 0;
+            c[CTL_RUN] = 1;
+            c[CTL_MODE] = MODE_NEWPASS;
+            c[CTL_PASSES + stage] = 1;
+            c[CTL_INNER + stage] = 1;
+        }
+    }
+    if ((int)blockIdx.x >= npad) return;
     const Item it = items[xcd_tile(blockIdx.x, gridDim.x)];
     const Inst& I = insts[it.inst];
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
@@ -1072,26 +1093,6 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     }, c1.run | c1.any_inadm | c1.mode);
 }
 
-// Start of a stage's MOOD loop, one thread per instance (inactive
-// instances stay out of the loop); also empties the active list the first
-// pass's scan writes.
-__global__ void k_begin(const Inst* __restrict__ insts, int ninst, int stage, int32_t* __restrict__ scan_count) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *reinterpret_cast<unsigned long long*>(scan_count) = 0;  // both counts of list 0
-    if (k >= ninst) return;
-    int32_t* c = insts[k].ctl;
-    if (!insts[k].active) {
-        c[CTL_RUN] = 0;
-        return;
-    }
-    c[CTL_ANY_INADM] = 0;
-    c[CTL_ANY_LOWER] = 0;
-    c[CTL_RUN] = 1;
-    c[CTL_MODE] = MODE_NEWPASS;
-    c[CTL_PASSES + stage] = 1;
-    c[CTL_INNER + stage] = 1;
-}
-
 // End of a MOOD iteration, one block.  Each instance still in its loop
 // decides from the flags of the iteration just run whether it continues
 // the inner admissibility loop (MODE_INNER), starts a new MOOD pass
@@ -1500,13 +1501,9 @@ void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st) 
                                  sizeof(double) * (kMaxR + 1) * 4 * set, hipMemcpyHostToDevice, st);
 }
 
-void launch_stage_init(const LaunchSet& L, hipStream_t st) {
-    if (L.npad <= 0) return;
-    hipLaunchKernelGGL(k_stage_init, g1(L.npad), dim3(256), 0, st, L.insts, L.pad);
-}
-
-void launch_begin(const LaunchSet& L, int stage, hipStream_t st) {
-    hipLaunchKernelGGL(k_begin, dim3((L.ninst + 63) / 64), dim3(64), 0, st, L.insts, L.ninst, stage, L.act_count);
+void launch_stage_init(const LaunchSet& L, int stage, hipStream_t st) {
+    hipLaunchKernelGGL(k_stage_init, g1(L.npad > 0 ? L.npad : 1), dim3(256), 0, st, L.insts, L.pad, L.npad, L.ninst,
+                       stage, L.act_count);
 }
 
 void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t seq, hipStream_t st) {

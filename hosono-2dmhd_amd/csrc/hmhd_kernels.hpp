@@ -9,9 +9,9 @@
 namespace hmhd {
 
 void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st);
-void launch_stage_init(const LaunchSet& L, hipStream_t st);
-// start of a stage's MOOD loop (the first pass's scan writes list 0)
-void launch_begin(const LaunchSet& L, int stage, hipStream_t st);
+// start of stage `stage`: the MOOD loop's control words (the first pass's
+// scan writes list 0) and the flags' stage set-up
+void launch_stage_init(const LaunchSet& L, int stage, hipStream_t st);
 // end of a MOOD iteration whose scan wrote list w: loop decisions, and the
 // control words copied to `snap` (a device address of pinned host memory),
 // whose sequence word is then set to `seq`

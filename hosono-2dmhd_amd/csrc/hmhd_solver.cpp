@@ -552,8 +552,7 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
 // Nothing here reads dt or changes the scheme.
 void Engine::first_pass_faces(int s) {
     const TileList all{L.pad, L.npad_dev, 1};
-    launch_begin(L, s, st_);
-    launch_stage_init(L, st_);
+    launch_stage_init(L, s, st_);
     prof_begin(PF_FACE_FULL);
     launch_face_groups(all, L.npad, true, s);
     prof_end(PF_FACE_FULL);
