@@ -25,7 +25,8 @@ namespace hmhd {
 enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
 enum : int { NCONS = 8 };
 // ctl words: flags the kernels raise, the MOOD-loop state k_stage_init and
-// k_finish keep on the device, and per-stage pass / inner-iteration counters
+// the loop decisions (in k_tile_scan's launch) keep on the device, and
+// per-stage pass / inner-iteration counters
 enum : int {
     CTL_ANY_INADM = 0,  // an inadmissible reconstructed state was found
     CTL_ANY_LOWER = 1,  // detect lowered a cell

@@ -1129,20 +1129,6 @@ __device__ __forceinline__ void publish_snapshot(int32_t* __restrict__ snap, int
     }
 }
 
-__global__ __launch_bounds__(256) void k_finish(int ninst, int stage, int32_t* __restrict__ ctl, int w,
-                                                int32_t* __restrict__ snap, uint32_t seq) {
-    loop_decisions(ctl, ninst, stage);
-    if (threadIdx.x == 0) {
-        int32_t* cnt = ctl + CTL_N * ninst;
-        snap[CTL_N * ninst + w] = cnt[2 * w];
-        snap[CTL_N * ninst + 3] = cnt[2 * w + 1];
-        *reinterpret_cast<unsigned long long*>(cnt + 2 * (1 - w)) = 0;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < CTL_N * ninst; t += blockDim.x) snap[t] = ctl[t];
-    publish_snapshot(snap, ninst, seq);
-}
-
 // Active tiles of the next MOOD iteration: every quantity a pass writes
 // (inner lowering, reconstruction, fluxes, update, detect bookkeeping) lies
 // within two cells of a cell flagged rc, rcF or inadm, so a tile takes part
@@ -1152,13 +1138,14 @@ __global__ __launch_bounds__(256) void k_finish(int ninst, int stage, int32_t* _
 // with one atomic, so the list count sees one update per block
 constexpr int kScanTiles = 4;
 //
-// With `snap` set, one more block ends the MOOD iteration as k_finish does
-// (the decisions read only flags earlier kernels set, and no scan or sync
-// block reads a word they write): the decisions, the control words copied
-// to `snap`, and, in snap's count slot w and word 3, the lengths of the
-// lists this iteration consumed (list 1 - w, complete since the previous
-// scan, and its face list), whose
-// count it then clears for the next scan; the sequence word `seq` last.
+// One more block ends the MOOD iteration (the decisions read only flags
+// earlier kernels set, and no scan or sync block reads a word they write):
+// the decisions, the control words copied to `snap`, and, in snap's count
+// slot w and word 3, the lengths of the lists this iteration consumed (list
+// 1 - w, complete since the previous scan, and its face list), whose count
+// it then clears for the next scan; the sequence word `seq` last.  After a
+// first pass, which consumed the fixed list of every tile, the count slots
+// hold nothing the host uses.
 template <bool ONE>
 __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
                                                                const Item* __restrict__ items, int npad,
@@ -1506,9 +1493,6 @@ void launch_stage_init(const LaunchSet& L, int stage, hipStream_t st) {
                        stage, L.act_count);
 }
 
-void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t seq, hipStream_t st) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, L.ninst, stage, L.ctl, w, snap, seq);
-}
 
 // the single-solve variant of a kernel when the table holds one solve
 #define HMHD_LAUNCH1(K, grid, block, ...)                                                   \

@@ -12,15 +12,13 @@ void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st);
 // start of stage `stage`: the MOOD loop's control words (the first pass's
 // scan writes list 0) and the flags' stage set-up
 void launch_stage_init(const LaunchSet& L, int stage, hipStream_t st);
-// end of a MOOD iteration whose scan wrote list w: loop decisions, and the
-// control words copied to `snap` (a device address of pinned host memory),
-// whose sequence word is then set to `seq`
-void launch_finish(const LaunchSet& L, int stage, int w, int32_t* snap, uint32_t seq, hipStream_t st);
 // `w`: which of the two active lists the scan of this iteration writes
 // stamp 0 scans every tile; otherwise only tiles marked with `stamp`.
-// With `snap` set the scan also ends the iteration as launch_finish does,
-// except that snap's count slot w (and word 3) get the lengths of the lists
-// the iteration consumed (list 1 - w and its face list)
+// The scan also ends the MOOD iteration: loop decisions, and the control
+// words copied to `snap` (a device address of pinned host memory) with, in
+// its count slot w (and word 3), the lengths of the lists the iteration
+// consumed (list 1 - w and its face list), then its sequence word set to
+// `seq`.
 // The periodic MOOD sync runs in the same launch (blocks ahead of the scan).
 void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
                       hipStream_t st);

@@ -390,7 +390,6 @@ int Engine::commit() {
                         "hipMemcpyAsync(npad)")))
         return rc;
     L.marks = d_marks_;
-    est_act_ = L.npad;
     L.cflpart = d_part_;
     L.one = n == 1 ? 1 : 0;
     L.inst1 = n == 1 ? table[0] : Inst{};
@@ -467,14 +466,6 @@ void Engine::prof_collect(bool wait_all) {
 }
 
 // ------------------------------------------------------------------ //
-
-// end of the MOOD iteration whose scan wrote list w in stage s: the loop
-// decisions, and the control words and both list counts written to host
-// slot w, ready when event w has fired
-int Engine::snapshot(int s, int w) {
-    launch_finish(L, s, w, d_hctl_ + w * snap_stride(), next_seq(w), st_);
-    return hip_check(hipGetLastError(), "kernel launch");
-}
 
 // the sequence number the next snapshot into slot w carries (never 0)
 uint32_t Engine::next_seq(int w) {
@@ -561,9 +552,10 @@ void Engine::first_pass_faces(int s) {
 // One RK stage: the MOOD loop of src/timeStep.f90:86-128 for every active
 // solve.  The loop runs one iteration ahead of the host: iteration it + 1
 // is enqueued before the host has read whether iteration it left work.  A
-// speculative iteration after the last needed one is a no-op (k_finish has
-// ended every solve's loop, and every kernel then returns), so the results
-// and the pass counts are those of the plain loop.
+// speculative iteration after the last needed one is a no-op (the loop
+// decisions have ended every solve's loop, and no kernel then changes the
+// state or its flags), so the results and the pass counts are those of the
+// plain loop.
 int Engine::run_stage(int s, bool pre) {
     const int n = (int)inst_.size();
     const TileList all{L.pad, L.npad_dev, 1};
@@ -575,13 +567,15 @@ int Engine::run_stage(int s, bool pre) {
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
     launch_lower_detect(L, all, L.npad, st_);
-    launch_scan_sync(L, 0, 0, s, nullptr, 0, st_);
+    launch_scan_sync(L, 0, 0, s, d_hctl_, next_seq(0), st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;
-    if ((rc = snapshot(s, 0))) return rc;
+    // Grids: iterations 1 and 2 read lists whose counts the host has not
+    // seen (the first pass's list has every tile while rc and rcF are set
+    // everywhere, until the first update); later ones the latest count.
     auto grid_for = [&](int listed) { return std::min(L.npad, listed + listed / 2 + 16); };
-    if ((rc = enqueue_iteration(s, 1, grid_for(est_act_)))) return rc;
+    if ((rc = enqueue_iteration(s, 1, L.npad))) return rc;
     for (int it = 0;; it++) {
         if ((rc = read_snapshot(it & 1, it > 0))) return rc;
         // every continuing iteration lowers at least one scheme value, so
@@ -595,11 +589,9 @@ int Engine::run_stage(int s, bool pre) {
         bool more = false;
         for (int k = 0; k < n; k++)
             if (ctl_[(size_t)CTL_N * k + CTL_RUN]) more = true;
-        if (it == 0) est_act_ = act_n_;
         if (!more) break;  // iteration it + 1, already enqueued, does nothing
-        // iteration it + 2 reads the list iteration it + 1 writes; size its
-        // grid from the latest count the host has seen
-        if ((rc = enqueue_iteration(s, it + 2, grid_for(act_n_)))) return rc;
+        // iteration it + 2 reads the list iteration it + 1 writes
+        if ((rc = enqueue_iteration(s, it + 2, it == 0 ? L.npad : grid_for(act_n_)))) return rc;
     }
     if (record_schemes) {
         size_t off = 0;

@@ -93,7 +93,6 @@ class Engine {
     int hip_check(hipError_t e, const char* what);
     int ensure_stream();
     int commit();
-    int snapshot(int s, int w);
     int read_snapshot(int w, bool visited);
     int enqueue_iteration(int s, int it, int grid);
     int run_stage(int s, bool pre);
@@ -158,7 +157,6 @@ class Engine {
     LaunchSet L{};
     int act_n_ = 0;    // length of the active list in the latest snapshot
     int32_t npad_val_ = 0;  // source of the device copy of L.npad
-    int est_act_ = 0;  // active tiles after the previous stage's first pass
     bool has_R_[kMaxR + 1] = {false};
     std::vector<int32_t> ctl_;
 
