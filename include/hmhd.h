@@ -85,7 +85,11 @@ int hmhd_set_gp(hmhd_t* h, const double* predvec, const double* quadw);
  * primitives on the first step; otherwise it derives V from U. */
 int hmhd_upload_state(hmhd_t* h, const double* U, const double* V);
 
-/* Replaces cfl_computedt (src/cfl.f90:14-52). */
+/* Replaces cfl_computedt (src/cfl.f90:14-52).  Before it returns it also
+ * enqueues the part of the next hmhd_advance that does not depend on dt
+ * (the first pass's face work of stage 0), which then overlaps the
+ * caller's handling of dt; hmhd_download in between is unaffected, and an
+ * upload or GP-table change in between discards that work. */
 int hmhd_cfl_dt(hmhd_t* h, double* dt);
 
 /* Replaces timeStep_advanceSolution (src/timeStep.f90:22-43): one SSP-RK3
