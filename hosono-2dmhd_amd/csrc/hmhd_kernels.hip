@@ -1372,9 +1372,21 @@ __global__ __launch_bounds__(256) void k_cfl_reduce(const Inst* __restrict__ ins
                                                     int32_t* __restrict__ seqp, uint32_t seq) {
     const Inst& I = insts[blockIdx.x];
     unsigned long long bx = 0, by = 0;
-    for (int b = I.cell_off + threadIdx.x; b < I.cell_off + I.cell_cnt; b += blockDim.x) {
-        bx = part[2 * b] > bx ? part[2 * b] : bx;
-        by = part[2 * b + 1] > by ? part[2 * b + 1] : by;
+    // eight partials a thread in flight at once
+    constexpr int K = 8;
+    const int end = I.cell_off + I.cell_cnt;
+    for (int b0 = I.cell_off + threadIdx.x; b0 < end; b0 += K * blockDim.x) {
+        ulonglong2 p[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int b = b0 + k * blockDim.x;
+            p[k] = b < end ? reinterpret_cast<const ulonglong2*>(part)[b] : ulonglong2{0, 0};
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            bx = p[k].x > bx ? p[k].x : bx;
+            by = p[k].y > by ? p[k].y : by;
+        }
     }
     bx = wave_max_u64(bx);
     by = wave_max_u64(by);
