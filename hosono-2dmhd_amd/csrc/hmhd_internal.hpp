@@ -36,6 +36,7 @@ enum : int {
     CTL_INNER = 7,      // [3] inner admissibility iterations per stage
     CTL_SYNC = 10,      // this pass ends with the periodic MOOD sync (set before the loop decisions)
     CTL_LOWER = 11,     // the next iteration is an inner-loop round, lowered in this pass's scan launch
+    CTL_FIRST = 12,     // no pass of this stage has run its update yet: rc and rcF are set everywhere
     CTL_N = 16
 };
 enum : int { MODE_NEWPASS = 0, MODE_INNER = 1 };

@@ -243,6 +243,7 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
             c[CTL_MODE] = MODE_NEWPASS;
             c[CTL_PASSES + stage] = 1;
             c[CTL_INNER + stage] = 1;
+            c[CTL_FIRST] = 1;
         }
     }
     if ((int)blockIdx.x >= npad) return;
@@ -778,6 +779,7 @@ __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ i
             const Ctl c = ONE ? c1 : ctl_of(I.ctl);
             I.ctl[CTL_SYNC] = sync_due(I, c) ? 1 : 0;
             I.ctl[CTL_LOWER] = c.run != 0 && c.any_inadm != 0 ? 1 : 0;
+            if (pass_ready(c)) I.ctl[CTL_FIRST] = 0;  // this launch replaces rc and rcF
         }
     }
     for_tiles(tl, [&](const Item it) {
@@ -1187,7 +1189,12 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
     const Item it = ONE ? Item{0, (int16_t)(tc % one.ntx), (int16_t)(tc / one.ntx)} : items[tc];
     const Inst& I = inst_at<ONE>(insts, one, it.inst);
     bool any = false, anyf = false;
-    if (t < npad && (stamp == 0 || marks[I.tile_off + it.ty * I.ntx + it.tx] == stamp)) {
+    // A single solve before its first update has rc and rcF set everywhere:
+    // every tile is active, and takes its own index in the list, without
+    // the atomic (with one per block, a full list cost as much as all the
+    // rest of the scan)
+    const bool idlist = ONE && I.ctl[CTL_FIRST] != 0;
+    if (t < npad && (idlist || stamp == 0 || marks[I.tile_off + it.ty * I.ntx + it.tx] == stamp)) {
         const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
         constexpr int W = kCellT + 4, NR = (W * W + 63) / 64;
         // rc and rcF of every cell of the footprint, read at once (clamped,
@@ -1212,7 +1219,7 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         const int dx = lane % 3 - 1, dy = lane / 3 - 1;
         const uint32_t need = 1u | (dx < 0 ? 4u : 0u) | (dx > 0 ? 2u : 0u) | (dy < 0 ? 16u : 0u) | (dy > 0 ? 8u : 0u);
         anyf = tin && (tf & need) == need;
-        any = anyf;
+        any = anyf || idlist;
 #pragma unroll
         for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
         // The sync of this launch may not have reached the footprint's
@@ -1256,13 +1263,13 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
         }
     }
     if (lane == 0) {
-        hit[w] = hw ? 1 : 0;
+        hit[w] = hw && !idlist ? 1 : 0;
         hitf[w] = hf ? 1 : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         // both lists' counts in one 64-bit atomic (each below 2^31)
-        unsigned long long n = 0;
+        unsigned long long n = idlist && sb == 0 ? (unsigned long long)npad : 0ull;
 #pragma unroll
         for (int k = 0; k < kScanTiles; k++) n += (unsigned long long)hit[k] | (unsigned long long)hitf[k] << 32;
         const unsigned long long b = n ? atomicAdd(reinterpret_cast<unsigned long long*>(act_count), n) : 0;
@@ -1276,7 +1283,7 @@ __global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __res
             r += hit[k];
             rf += hitf[k];
         }
-        if (hw) act[base + r] = it;
+        if (hw) act[idlist ? t : base + r] = it;
         if (hf) actf[basef + rf] = it;
     }
 }
