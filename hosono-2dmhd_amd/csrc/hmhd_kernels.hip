@@ -1078,13 +1078,17 @@ template <bool ONE>
 __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
                                                 int stage, uint32_t* __restrict__ marks, uint32_t stamp) {
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
+    // a single solve before its first update, in a pass that is not ready
+    // either: the scan that ends the iteration lists every tile (it reads
+    // no candidate marks), so none are written
+    const bool nomarks = ONE && one.ctl[CTL_FIRST] != 0 && !pass_ready(c1);
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
         if (stamp != 0) {
             // a later iteration's active tiles: this is the kernel that
             // visits all of them, so it marks the scan candidates
-            if (cw.run) mark_candidates(I, it, marks, stamp);
+            if (cw.run && !nomarks) mark_candidates(I, it, marks, stamp);
         } else {
             // the first pass, every tile: the stage set-up's scheme = R
             // (src/timeStep.f90:79), read first here
