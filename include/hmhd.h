@@ -7,7 +7,8 @@
  *                                                     src/mhd_driver.f90:81
  * plus the set-up calls simulation_init / GP_init / grid_init
  * (src/mhd_driver.f90:40-52).  The entry points below replace exactly those
- * calls; a Fortran driver binds them with iso_c_binding.
+ * calls; a Fortran driver binds them with iso_c_binding (the interfaces in
+ * hosono-2dmhd_amd/fortran/hmhd_c.f90).
  *
  * Array layouts are the reference's Fortran layouts on the padded grid
  * (NGC = R + 2 guard cells per side, src/grid.f90:71-81):
