@@ -6,6 +6,8 @@ bar is bit-for-bit equality of U, V, the per-stage MOOD scheme masks, dt and
 the MOOD pass counts.  The oracle itself is bit-identical to the reference's
 Fortran (tests/test_oracle_vs_reference.py).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -101,6 +103,69 @@ def test_steps_bitwise(name, steps):
     for n in range(steps):
         _step_both(orc, gpu)
         _compare_state(orc, gpu, f"{name} step {n + 1}")
+
+
+@pytest.mark.parametrize("name,n,R", [
+    ("ot", 6, 1), ("ot", 8, 2), ("ot", 10, 3),  # N = 2 NGC: guard and image bands meet
+    ("blast", 6, 1), ("blast", 8, 2),           # one tile holds the whole padded grid
+    ("sedov", 10, 3),                           # two ragged tiles a side, several passes
+])
+def test_smallest_grids_bitwise(name, n, R):
+    cfg, g, orc, gpu = _pair(name, n=n, R=R)
+    for k in range(6):
+        _step_both(orc, gpu)
+        _compare_state(orc, gpu, f"{name} N={n} R={R} step {k + 1}")
+
+
+def _totals(U, ngc):
+    """Interior sums of the eight conserved variables, summed exactly
+    (math.fsum: a plain floating-point sum over 10^6 cells carries errors
+    near 1e-11 of the total), and the sums of their moduli."""
+    inner = U[ngc:-ngc, ngc:-ngc, :]
+    exact = np.array([math.fsum(inner[:, :, v].ravel()) for v in range(inner.shape[2])])
+    return exact, np.abs(inner).sum(axis=(0, 1))
+
+
+def test_full_size_first_steps_bitwise():
+    """The benchmarked workload itself, Orszag-Tang 1024^2 at R = 2: the
+    first two steps, bit for bit."""
+    cfg, g, orc, gpu = _pair("ot", n=1024)
+    for k in range(2):
+        _step_both(orc, gpu)
+        _compare_state(orc, gpu, f"ot 1024^2 step {k + 1}")
+    gpu.close()
+
+
+def test_full_size_late_time_from_gpu_state():
+    """Orszag-Tang 1024^2 on the GPU alone to t = 0.2 (shocks formed, many
+    MOOD passes per stage).  The periodic totals of the conserved variables
+    keep their initial values to rounding (the update is in flux form), and
+    from the GPU's state the oracle and the GPU take two more steps, bit for
+    bit."""
+    cfg, g, U0, V0 = cases.make("ot", n=1024)
+    gpu = rt.Solver(cases.runtime_config(cfg))
+    gpu.upload(U0, V0)
+    t = 0.0
+    while t < 0.2:
+        dt = rt.valid_time_step(gpu.cfl_computedt(), t, 0.2)
+        gpu.advance_solution(dt)
+        t += dt
+    Ug, Vg, _ = gpu.download()
+    ngc = cfg.R + 2
+    s0, a0 = _totals(U0, ngc)
+    s1, a1 = _totals(Ug, ngc)
+    drift = np.abs(s1 - s0) / np.maximum(a0, 1.0)
+    # the oracle's own run (OT 128^2 to t = 0.2) moves them by 1.5e-17 at most
+    assert drift.max() < 1e-14, f"conserved totals drift by {drift}"
+    orc = O.Oracle(cfg)
+    orc.set_state(Ug, Vg)
+    passes = 0
+    for k in range(2):
+        _step_both(orc, gpu)
+        _compare_state(orc, gpu, f"ot 1024^2 from t=0.2, step {k + 1}")
+        passes += int(gpu.stats()[0].sum())
+    assert passes > 6, "expected MOOD passes beyond the first at t = 0.2"
+    gpu.close()
 
 
 def test_late_time_ot_with_heavy_mood_activity():
