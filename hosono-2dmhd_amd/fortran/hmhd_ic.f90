@@ -12,6 +12,10 @@
 !                         IC_magy, IC_magz), and IC_energy of thermal energy
 !                         deposited in the 3 x 3 interior cells centred on
 !                         cell (Nx/2, Ny/2) (0-based)
+! With IC_noiseSeed = s (a non-negative integer) in the .par file, rho, v,
+! p and B of the interior are then multiplied by (1 + IC_noiseAmp u), with
+! IC_noiseAmp 0.01 unless given and u in [-1, 1) from a counter-based hash
+! (noise_u below), the same values hosono_2dmhd_amd.ics.perturb uses.
 ! Then, as initialCondition_set does (src/initialCondition.f90:62-78): eint
 ! and gamma on the interior, U = prim2cons(V) (src/convert.f90:159-196) and
 ! the guard-cell fills of U and V (src/boundaryConditions.f90:118-187).
@@ -24,8 +28,9 @@ module hmhd_ic
     real(8), parameter :: pi = acos(-1.0d0)
     integer, parameter :: dens_v = 1, velx_v = 2, vely_v = 3, velz_v = 4, pres_v = 5, &
                           magx_v = 6, magy_v = 7, magz_v = 8, eint_v = 9, gamm_v = 10
+    integer(8), parameter :: mask32 = 4294967295_8
 
-    public :: ic_set, prim2cons, bc_fill
+    public :: ic_set, prim2cons, bc_fill, hash32
 
 contains
 
@@ -75,6 +80,7 @@ contains
             write(*,*) "=========================================================================="
             stop
         end select
+        if (par_has(parfile, "IC_noiseSeed")) call add_noise(parfile, nx, s, ex, ey, V)
 
         do j = s, ey
             do i = s, ex
@@ -86,6 +92,61 @@ contains
         call bc_fill(U, nx, ny, ngc, periodic)
         call bc_fill(V, nx, ny, ngc, periodic)
     end subroutine ic_set
+
+    ! rho, v, p and B of the interior times (1 + amp u), u = noise_u(seed, k, c)
+    ! for the k-th of those eight variables (k = 0..7) and interior cell
+    ! c = jj nx + ii (0-based)
+    subroutine add_noise(parfile, nx, s, ex, ey, V)
+        character(len=*), intent(in) :: parfile
+        integer, intent(in) :: nx, s, ex, ey
+        real(8), intent(inout) :: V(:, :, :)
+        integer :: seed, k, i, j
+        integer(8) :: base, c
+        real(8) :: amp, u
+
+        seed = par_int(parfile, "IC_noiseSeed")
+        if (seed < 0) then
+            write(*,*) "IC_noiseSeed must be a non-negative integer"
+            stop
+        end if
+        amp = 0.01d0
+        if (par_has(parfile, "IC_noiseAmp")) amp = par_real(parfile, "IC_noiseAmp")
+        do k = 0, 7
+            base = hash32(iand(8_8 * seed + k, mask32))
+            do j = s, ey
+                do i = s, ex
+                    c = int(j - s, 8) * nx + (i - s)
+                    u = real(hash32(iand(c + base, mask32)), 8) * 2.0d0**(-31) - 1.0d0
+                    V(dens_v + k, i, j) = V(dens_v + k, i, j) * (1.0d0 + amp * u)
+                end do
+            end do
+        end do
+    end subroutine add_noise
+
+    ! lowbias32, C. Wellons' 32-bit integer hash ("Prospecting for hash
+    ! functions", 2018): x ^= x >> 16; x *= 0x7feb352d; x ^= x >> 15;
+    ! x *= 0x846ca68b; x ^= x >> 16, modulo 2^32, in 64-bit integers
+    integer(8) function hash32(x0)
+        integer(8), intent(in) :: x0
+        integer(8) :: x
+
+        x = iand(x0, mask32)
+        x = ieor(x, shiftr(x, 16))
+        x = mulmod32(x, 2146121005_8)
+        x = ieor(x, shiftr(x, 15))
+        x = mulmod32(x, 2221713035_8)
+        hash32 = ieor(x, shiftr(x, 16))
+    end function hash32
+
+    ! x c modulo 2^32 for x, c in [0, 2^32), from x c = x c_hi 2^16 + x c_lo:
+    ! every product stays below 2^48
+    integer(8) function mulmod32(x, c)
+        integer(8), intent(in) :: x, c
+        integer(8) :: t
+
+        t = iand(x * shiftr(c, 16), 65535_8)
+        mulmod32 = iand(shiftl(t, 16) + x * iand(c, 65535_8), mask32)
+    end function mulmod32
 
     ! src/convert.f90:159-196
     subroutine prim2cons(Vc, gamma, force_hydro, Uc)

@@ -13,7 +13,9 @@ Restates, in numpy with the reference's operation order:
 * three workload ICs the reference cannot express: the Orszag-Tang vortex
   with rho = gamma^2, p = gamma, B = (-sin 2 pi y, sin 4 pi x)/sqrt(4 pi);
   the MHD blast with B = (1, 1, 0)/sqrt(2); and a Sedov blast with unit
-  energy deposited in the central 3x3 cells and B = (0.5, 0, 0).
+  energy deposited in the central 3x3 cells and B = (0.5, 0, 0);
+* seeded multiplicative noise on rho, v, p and B (`perturb`), from a
+  counter-based hash that the Fortran driver shares.
 
 Arrays follow the reference's Fortran layout U(8, i, j) / V(10, i, j) on the
 padded grid, held in numpy as C-ordered arrays of shape (my, mx, 8) and
@@ -236,13 +238,47 @@ def sedov_reference(grid: Grid, gamma: float, centre=(0.0, 0.0), force_hydro: bo
     return finish_state(V, grid, gamma, BC_OUTFLOW, force_hydro)
 
 
-def perturb(V: np.ndarray, grid: Grid, gamma: float, bc: int, seed: int, amplitude: float = 0.01):
-    """Multiply rho, v, p and B of the interior by (1 + amplitude * u),
-    u ~ U[-1, 1) from a seeded generator, then rebuild U and the guards."""
-    rng = np.random.default_rng(seed)
+# lowbias32, C. Wellons' 32-bit integer hash ("Prospecting for hash
+# functions", 2018): x ^= x >> 16; x *= 0x7feb352d; x ^= x >> 15;
+# x *= 0x846ca68b; x ^= x >> 16, all modulo 2^32.
+_MASK32 = np.uint64(0xFFFFFFFF)
+
+
+def hash32(x):
+    """lowbias32 of x (integers in [0, 2^32), any numpy shape)."""
+    x = np.asarray(x, dtype=np.uint64) & _MASK32
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & _MASK32
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & _MASK32
+    x ^= x >> np.uint64(16)
+    return x
+
+
+def noise_uniform(seed: int, k: int, ny: int, nx: int) -> np.ndarray:
+    """Seeded noise in [-1, 1) on the ny x nx interior, stream k of 8:
+    u(c) = h * 2^-31 - 1 with h = hash32(c + hash32(8 seed + k)) mod 2^32
+    and c = jj nx + ii the 0-based interior cell index.  Counter-based, so
+    the Fortran driver (hmhd_ic.f90) reproduces it bit for bit."""
+    base = hash32((8 * seed + k) & 0xFFFFFFFF)
+    c = np.arange(ny * nx, dtype=np.uint64).reshape(ny, nx)
+    h = hash32((c + base) & _MASK32)
+    return h.astype(np.float64) * 2.0 ** -31 - 1.0
+
+
+NOISE_VARS = (DENS, VELX, VELY, VELZ, PRES, MAGX, MAGY, MAGZ)
+
+
+def perturb(V: np.ndarray, grid: Grid, gamma: float, bc: int, seed: int, amplitude: float = 0.01,
+            force_hydro: bool = False):
+    """Multiply rho, v, p and B of the interior by (1 + amplitude * u), u
+    from noise_uniform (stream k for the k-th of those eight variables),
+    then rebuild eint, U and the guards."""
+    if seed < 0:
+        raise ValueError("the noise seed is a non-negative integer")
     js, is_ = grid.interior()
     V = V.copy()
-    for var in (DENS, VELX, VELY, VELZ, PRES, MAGX, MAGY, MAGZ):
-        u = rng.uniform(-1.0, 1.0, size=(grid.ny, grid.nx))
+    for k, var in enumerate(NOISE_VARS):
+        u = noise_uniform(seed, k, grid.ny, grid.nx)
         V[js, is_, var] *= 1.0 + amplitude * u
-    return finish_state(V, grid, gamma, bc)
+    return finish_state(V, grid, gamma, bc, force_hydro)

@@ -70,9 +70,18 @@ IC_CASES = [
                  "grid_yBeg": "-0.5", "grid_yEnd": "0.5", "GP_maxRadius": "3"},
      lambda g, gam: ics.sedov_reference(g, gam), ics.BC_OUTFLOW, True),
 ]
+# the same with seeded noise (IC_noiseSeed, IC_noiseAmp)
+IC_CASES += [
+    ("OrszagTang2dWorkload", {"IC_noiseSeed": "7"},
+     lambda g, gam: ics.perturb(ics.orszag_tang(g, gam)[1], g, gam, ics.BC_PERIODIC, 7), ics.BC_PERIODIC, False),
+    ("sedov2dCells", dict(IC_CASES[3][1], IC_noiseSeed="12345", IC_noiseAmp="0.05"),
+     lambda g, gam: ics.perturb(ics.sedov(g, gam)[1], g, gam, ics.BC_OUTFLOW, 12345, 0.05), ics.BC_OUTFLOW,
+     False),
+]
+IC_IDS = [c[0] + ("+noise" if "IC_noiseSeed" in c[1] else "") for c in IC_CASES]
 
 
-@pytest.mark.parametrize("ic_type,extra,make,bc,hydro", IC_CASES, ids=[c[0] for c in IC_CASES])
+@pytest.mark.parametrize("ic_type,extra,make,bc,hydro", IC_CASES, ids=IC_IDS)
 def test_initial_conditions_match_the_python_restatement(tmp_path, ic_type, extra, make, bc, hydro):
     """--ic-only writes output 0 without a device; its primitives and
     conserved state equal hosono_2dmhd_amd.ics bit for bit."""
@@ -98,6 +107,43 @@ def test_initial_conditions_match_the_python_restatement(tmp_path, ic_type, extr
     magp = (V[js, is_, 5] ** 2 + V[js, is_, 6] ** 2 + V[js, is_, 7] ** 2) / 2
     assert np.array_equal(d["magp"], magp)
     assert np.array_equal(d["ener"], ener + magp)
+
+
+def _lowbias32(x):
+    """The hash on Python integers, one value at a time."""
+    m = 0xFFFFFFFF
+    x &= m
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & m
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & m
+    return x ^ (x >> 16)
+
+
+def test_noise_generator():
+    """ics.hash32 (numpy, vectorised) equals the hash computed on Python
+    integers; the noise lies in [-1, 1) with mean ~0 and variance ~1/3, and
+    its streams differ by seed and variable."""
+    xs = [0, 1, 2, 3, 255, 65535, 65536, 0x7FFFFFFF, 0x80000000, 0xDEADBEEF, 0xFFFFFFFF]
+    assert [int(h) for h in ics.hash32(np.array(xs))] == [_lowbias32(x) for x in xs]
+    xs = np.random.default_rng(0).integers(0, 2 ** 32, 1000)
+    assert [int(h) for h in ics.hash32(xs)] == [_lowbias32(int(x)) for x in xs]
+    u = ics.noise_uniform(7, 0, 128, 128)
+    assert u.min() >= -1.0 and u.max() < 1.0
+    assert abs(u.mean()) < 0.02 and abs(u.var() - 1.0 / 3.0) < 0.02
+    assert np.array_equal(u, ics.noise_uniform(7, 0, 128, 128))
+    assert not np.array_equal(u, ics.noise_uniform(8, 0, 128, 128))
+    assert not np.array_equal(u, ics.noise_uniform(7, 1, 128, 128))
+    # the value at interior cell (jj, ii) depends only on jj nx + ii
+    assert np.array_equal(ics.noise_uniform(7, 2, 4, 6).ravel(), ics.noise_uniform(7, 2, 6, 4).ravel())
+
+
+def test_negative_noise_seed_stops(tmp_path):
+    r = run_driver(write_par(tmp_path, "neg", IC_type="'OrszagTang2dWorkload'", IC_noiseSeed="-1"), "--ic-only")
+    assert r.returncode == 0 and "IC_noiseSeed must be a non-negative integer" in r.stdout
+    with pytest.raises(ValueError):
+        g = ics.Grid(8, 8, R=2)
+        ics.perturb(ics.orszag_tang(g)[1], g, 5.0 / 3.0, ics.BC_PERIODIC, -1)
 
 
 def test_missing_parameter_stops_with_the_reference_message(tmp_path):
