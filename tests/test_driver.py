@@ -170,26 +170,36 @@ def test_comments_and_first_match_follow_the_reference(tmp_path):
     assert outfile.read_output(tmp_path / "c_00000.hmhd")["nx"] == 24
 
 
+RUN_CASES = [
+    ("ot", {"IC_type": "'OrszagTang2dWorkload'"}, ics.BC_PERIODIC, (0.0, 1.0, 0.0, 1.0)),
+    ("sedov_noise", dict(IC_CASES[3][1], IC_type="'sedov2dCells'", IC_noiseSeed="3"), ics.BC_OUTFLOW,
+     (-0.5, 0.5, -0.5, 0.5)),
+]
+
+
 @pytest.mark.gpu
-def test_driver_run_is_bit_identical_to_the_oracle_loop(tmp_path):
-    """Five steps of OT 24 x 20 with an output every second step: every
-    output equals the oracle's driver loop from the same initial state."""
-    par = write_par(tmp_path, "run", IC_type="'OrszagTang2dWorkload'", sim_tmax="1.0", sim_nstepmax="5")
+@pytest.mark.parametrize("name,extra,bc,box", RUN_CASES, ids=[c[0] for c in RUN_CASES])
+def test_driver_run_is_bit_identical_to_the_oracle_loop(tmp_path, name, extra, bc, box):
+    """Five steps on 24 x 20 (the workload OT, and the Sedov blast with
+    seeded noise) with an output every second step: every output equals the
+    oracle's driver loop from the same initial state."""
+    par = write_par(tmp_path, "run", sim_tmax="1.0", sim_nstepmax="5", **extra)
     r = run_driver(par)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     files = sorted(tmp_path.glob("run_*.hmhd"))
     assert [f.name for f in files] == [f"run_{k:05d}.hmhd" for k in range(4)]
     d0 = outfile.read_output(files[0])
-    g = ics.Grid(24, 20, R=2)
+    g = ics.Grid(24, 20, *box, R=2)
     U = np.zeros((g.my, g.mx, 8))
     V = np.zeros((g.my, g.mx, 10))
     js, is_ = g.interior()
     U[js, is_] = d0["cons"]
     for k, name in enumerate(PRIMS):
         V[js, is_, k] = d0[name]
-    ics.bc_fill(U, g, ics.BC_PERIODIC)
-    ics.bc_fill(V, g, ics.BC_PERIODIC)
-    orc = O.Oracle(O.Config(24, 20, R=2, ell_over_delta=12.0, gamma=float(BASE["sim_gamma"])))
+    ics.bc_fill(U, g, bc)
+    ics.bc_fill(V, g, bc)
+    obc = O.BC_PERIODIC if bc == ics.BC_PERIODIC else O.BC_OUTFLOW
+    orc = O.Oracle(O.Config(24, 20, *box, R=2, ell_over_delta=12.0, gamma=float(BASE["sim_gamma"]), bc=obc))
     orc.set_state(U, V)
     t, done = 0.0, 0
     for step, f in zip((2, 4, 5), files[1:]):
