@@ -186,15 +186,18 @@ __device__ __forceinline__ void use_all(T... v) {
 // The tiles of list `tl` this block takes: entries b, b + G, ... below the
 // count.  The block's first entry, the count and the caller's `pin` value
 // (its control words) are loaded together, in one round trip; the launch
-// has at most as many blocks as the list has room for.
+// has at most as many blocks as the list has room for.  `idle`, a value
+// computed from those control words: the launch has nothing to do, and
+// the block leaves before walking its entries.
 template <class F>
-__device__ __forceinline__ void for_tiles(const TileList& tl, F&& body, int pin = 0) {
+__device__ __forceinline__ void for_tiles(const TileList& tl, F&& body, int pin = 0, bool idle = false) {
     const int b0 = blockIdx.x, G = gridDim.x;
     Item it = tl.items[tl.swz ? xcd_tile(b0, G) : b0];
     const int n = *tl.cnt;
     use_now(it.inst);
     use_now(it.tx);
     use_now(pin);
+    if (idle) return;
     for (int b = b0; b < n;) {
         body(it);
         b += G;
@@ -785,7 +788,7 @@ __global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ i
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         lower_detect_tile(I, ONE ? c1 : ctl_of(I.ctl), it);
-    }, c1.run | c1.any_inadm | c1.mode);
+    }, c1.run | c1.any_inadm | c1.mode, ONE && !pass_ready(c1));
 }
 
 // Periodic MOOD sync (src/boundaryConditions.f90:205-252) in one launch.
@@ -1096,7 +1099,7 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
             if (i < I.mx && j < I.my) I.scheme[(size_t)j * I.mx + i] = (uint8_t)I.R;
         }
         update_tile(I, cw, it, stage);
-    }, c1.run | c1.any_inadm | c1.mode);
+    }, c1.run | c1.any_inadm | c1.mode, stamp != 0 && nomarks);
 }
 
 // End of a MOOD iteration, one block.  Each instance still in its loop

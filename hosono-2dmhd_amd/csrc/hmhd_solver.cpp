@@ -516,8 +516,9 @@ void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
 
 // MOOD iteration it >= 1 of stage s: the active tiles listed by iteration
 // it - 1 (list (it - 1) & 1), its own scan writing list it & 1, then a
-// snapshot into host slot it & 1.  `grid` blocks stride over the list.
-int Engine::enqueue_iteration(int s, int it, int grid) {
+// snapshot into host slot it & 1.  The face kernel's `gface` blocks and
+// the update's and lowering's `gupd` blocks stride over their lists.
+int Engine::enqueue_iteration(int s, int it, int gface, int gupd) {
     const int r = (it - 1) & 1, w = it & 1;
     const TileList act{L.act[r], L.act_count + 2 * r, 0};
     const TileList actf{L.actf[r], L.act_count + 2 * r + 1, 0};
@@ -525,13 +526,13 @@ int Engine::enqueue_iteration(int s, int it, int grid) {
     if (++stamp_ == 0) stamp_ = 1;
     // (an inner-loop round's lowering ran in the previous scan launch)
     prof_begin(PF_FACE_FLUX);
-    launch_face_groups(actf, grid, false, s);
+    launch_face_groups(actf, gface, false, s);
     prof_end(PF_FACE_FLUX);
     prof_begin(PF_UPDATE);
-    launch_update(L, act, grid, s, stamp_, st_);
+    launch_update(L, act, gupd, s, stamp_, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
-    launch_lower_detect(L, act, grid, st_);
+    launch_lower_detect(L, act, gupd, st_);
     // the sync and the scan, which also ends the iteration (loop decisions, snapshot)
     launch_scan_sync(L, w, stamp_, s, d_hctl_ + w * snap_stride(), next_seq(w), st_);
     prof_end(PF_FLAGS);
@@ -575,9 +576,20 @@ int Engine::run_stage(int s, bool pre) {
     // seen (the first pass's list has every tile while rc and rcF are set
     // everywhere, until the first update); later ones the latest count.
     auto grid_for = [&](int listed) { return std::min(L.npad, listed + listed / 2 + 16); };
-    if ((rc = enqueue_iteration(s, 1, L.npad))) return rc;
+    // One block per tile, except where a single solve's previous step says
+    // the kernel has little to do: its same stage's pass first ran its
+    // update at iteration pr.  Iterations it <= pr were inner-loop rounds,
+    // whose face list holds only the tiles near an inadmissible cell, and
+    // before it the update and lowering found nothing to do (a single
+    // solve marks no candidates then).  A wrong guess costs time only: the
+    // blocks stride over whatever the list holds.
+    const int pr = L.one ? ready_pred_[s] : 0, quiet = std::min(L.npad, 512);
+    auto early = [&](int i) { return enqueue_iteration(s, i, pr >= i ? quiet : L.npad, pr > i ? quiet : L.npad); };
+    int ready = -1;  // this stage's pr
+    if ((rc = early(1))) return rc;
     for (int it = 0;; it++) {
         if ((rc = read_snapshot(it & 1, it > 0))) return rc;
+        if (L.one && ready < 0 && ctl_[CTL_FIRST] == 0) ready = it;
         // every continuing iteration lowers at least one scheme value, so
         // the loop ends; the cap only turns a defect into an error
         if (it > 64 * (kMaxR + 1) * 64) {
@@ -591,8 +603,9 @@ int Engine::run_stage(int s, bool pre) {
             if (ctl_[(size_t)CTL_N * k + CTL_RUN]) more = true;
         if (!more) break;  // iteration it + 1, already enqueued, does nothing
         // iteration it + 2 reads the list iteration it + 1 writes
-        if ((rc = enqueue_iteration(s, it + 2, it == 0 ? L.npad : grid_for(act_n_)))) return rc;
+        if ((rc = it == 0 ? early(2) : enqueue_iteration(s, it + 2, grid_for(act_n_), grid_for(act_n_)))) return rc;
     }
+    if (L.one) ready_pred_[s] = ready > 0 ? ready : 0;
     if (record_schemes) {
         size_t off = 0;
         for (auto& h : inst_) {

@@ -94,7 +94,7 @@ class Engine {
     int ensure_stream();
     int commit();
     int read_snapshot(int w, bool visited);
-    int enqueue_iteration(int s, int it, int grid);
+    int enqueue_iteration(int s, int it, int gface, int gupd);
     int run_stage(int s, bool pre);
     void first_pass_faces(int s);
     void prof_begin(int fam);
@@ -156,6 +156,9 @@ class Engine {
     size_t cap_stage_i_ = 0;
     LaunchSet L{};
     int act_n_ = 0;    // length of the active list in the latest snapshot
+    // A single solve: per RK stage, the MOOD iteration whose pass first ran
+    // its update, in the previous step (the grids of iterations 1 and 2)
+    int ready_pred_[3] = {0, 0, 0};
     int32_t npad_val_ = 0;  // source of the device copy of L.npad
     bool has_R_[kMaxR + 1] = {false};
     std::vector<int32_t> ctl_;
