@@ -1085,6 +1085,18 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
     // either: the scan that ends the iteration lists every tile (it reads
     // no candidate marks), so none are written
     const bool nomarks = ONE && one.ctl[CTL_FIRST] != 0 && !pass_ready(c1);
+    if (ONE && stamp == 0 && !pass_ready(c1)) {
+        // the first pass of a single solve whose pass is not ready: only
+        // the scheme's reset (below), by tile index, since the solve's
+        // tiles are its rows of tiles in order; any grid covers them
+        const int nt = one.ntx * ((one.my + kCellT - 1) / kCellT);
+        const int lx = threadIdx.x % kCellT, ly = threadIdx.x / kCellT;
+        for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+            const int i = (t % one.ntx) * kCellT + lx, j = (t / one.ntx) * kCellT + ly;
+            if (i < one.mx && j < one.my) one.scheme[(size_t)j * one.mx + i] = (uint8_t)one.R;
+        }
+        return;
+    }
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         const Ctl cw = ONE ? c1 : ctl_of(I.ctl);

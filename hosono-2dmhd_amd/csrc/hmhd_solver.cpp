@@ -563,11 +563,20 @@ int Engine::run_stage(int s, bool pre) {
     // iteration 0, the first pass: every tile, every face at the top radius
     // (its start already enqueued by cfl() when `pre`)
     if (!pre) first_pass_faces(s);
+    // One block per tile, except where a single solve's previous step says
+    // the kernel has little to do: its same stage's pass first ran its
+    // update at iteration pr.  Iterations it < pr found an inadmissible
+    // face side, so their update and lowering did nothing (apart from the
+    // first pass's scheme reset, and no candidate marks for a single
+    // solve), and iterations 1..pr were inner-loop rounds, whose face list
+    // holds only the tiles near an inadmissible cell.  A wrong guess costs
+    // time only: the blocks stride over whatever the list holds.
+    const int pr = L.one ? ready_pred_[s] : 0, quiet = std::min(L.npad, 512);
     prof_begin(PF_UPDATE);
-    launch_update(L, all, L.npad, s, 0, st_);
+    launch_update(L, all, pr > 0 ? quiet : L.npad, s, 0, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
-    launch_lower_detect(L, all, L.npad, st_);
+    launch_lower_detect(L, all, pr > 0 ? quiet : L.npad, st_);
     launch_scan_sync(L, 0, 0, s, d_hctl_, next_seq(0), st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
@@ -576,14 +585,6 @@ int Engine::run_stage(int s, bool pre) {
     // seen (the first pass's list has every tile while rc and rcF are set
     // everywhere, until the first update); later ones the latest count.
     auto grid_for = [&](int listed) { return std::min(L.npad, listed + listed / 2 + 16); };
-    // One block per tile, except where a single solve's previous step says
-    // the kernel has little to do: its same stage's pass first ran its
-    // update at iteration pr.  Iterations it <= pr were inner-loop rounds,
-    // whose face list holds only the tiles near an inadmissible cell, and
-    // before it the update and lowering found nothing to do (a single
-    // solve marks no candidates then).  A wrong guess costs time only: the
-    // blocks stride over whatever the list holds.
-    const int pr = L.one ? ready_pred_[s] : 0, quiet = std::min(L.npad, 512);
     auto early = [&](int i) { return enqueue_iteration(s, i, pr >= i ? quiet : L.npad, pr > i ? quiet : L.npad); };
     int ready = -1;  // this stage's pr
     if ((rc = early(1))) return rc;
