@@ -1167,8 +1167,11 @@ constexpr int kScanTiles = 4;
 // it then clears for the next scan; the sequence word `seq` last.  After a
 // first pass, which consumed the fixed list of every tile, the count slots
 // hold nothing the host uses.
+// Five waves per SIMD for the many-solve form (93 VGPRs, no spills): more
+// scan blocks in flight.  The single solve keeps the compiler's own choice
+// (four): at five it spills 14 VGPRs and runs slower.
 template <bool ONE>
-__global__ __launch_bounds__(64 * kScanTiles) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
+__global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
                                                                const Item* __restrict__ items, int npad,
                                                                Item* __restrict__ act, Item* __restrict__ actf,
                                                                int32_t* __restrict__ act_count,
