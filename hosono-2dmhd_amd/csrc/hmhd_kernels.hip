@@ -295,13 +295,19 @@ constexpr int face_lds_cells() {
 
 // Where in a tile the cell of an inadmissible face side lies, for the scan
 // of the tile and of its eight neighbours: bit 0 for any, then bits for a
-// cell within two cells of the tile's west, east, south and north edge.
+// cell within two cells of the tile's west, east, south and north edge
+// (BAD_W2 .. BAD_N2), then the same within four cells (BAD_W4 .. BAD_N4).
 // (rx, ry): the cell relative to the tile's origin; -1 for a cell of the
 // west / south neighbour, whose east / north side belongs to this tile's
 // face.
+enum : uint32_t {
+    BAD_ANY = 1u, BAD_W2 = 2u, BAD_E2 = 4u, BAD_S2 = 8u, BAD_N2 = 16u,
+    BAD_W4 = 32u, BAD_E4 = 64u, BAD_S4 = 128u, BAD_N4 = 256u, BAD_NBITS = 9
+};
 __device__ __forceinline__ uint32_t bad_bits(int rx, int ry) {
-    return 1u | (rx <= 1 ? 2u : 0u) | (rx >= kCellT - 2 ? 4u : 0u) | (ry <= 1 ? 8u : 0u) |
-           (ry >= kCellT - 2 ? 16u : 0u);
+    return BAD_ANY | (rx <= 1 ? BAD_W2 : 0u) | (rx >= kCellT - 2 ? BAD_E2 : 0u) | (ry <= 1 ? BAD_S2 : 0u) |
+           (ry >= kCellT - 2 ? BAD_N2 : 0u) | (rx <= 3 ? BAD_W4 : 0u) | (rx >= kCellT - 4 ? BAD_E4 : 0u) |
+           (ry <= 3 ? BAD_S4 : 0u) | (ry >= kCellT - 4 ? BAD_N4 : 0u);
 }
 // The end of a face tile: the bad_bits of the wave's active lanes go into
 // the tile's word with one atomic, and ANY_INADM is raised, by the first
@@ -310,7 +316,7 @@ __device__ __forceinline__ uint32_t bad_bits(int rx, int ry) {
 __device__ __forceinline__ void report_bad(const Inst& I, uint32_t* tbad, uint32_t bits) {
     uint32_t w = 0;
 #pragma unroll
-    for (int k = 0; k < 5; k++) w |= __ballot((bits >> k) & 1u) != 0 ? 1u << k : 0u;
+    for (int k = 0; k < BAD_NBITS; k++) w |= __ballot((bits >> k) & 1u) != 0 ? 1u << k : 0u;
     if (w && (int)(threadIdx.x & 63u) == __ffsll(__ballot(1)) - 1) {
         I.ctl[CTL_ANY_INADM] = 1;
         atomicOr(tbad, w);
@@ -1216,6 +1222,10 @@ __global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(cons
     // the atomic (with one per block, a full list cost as much as all the
     // rest of the scan)
     const bool idlist = ONE && I.ctl[CTL_FIRST] != 0;
+    // Before an inner-loop round (CTL_LOWER) this launch also lowers: the
+    // waves of the face list's tiles raise rc / rcF / rec within two cells
+    // of an inadmissible cell, concurrently with the other waves' reads.
+    const bool lower = I.ctl[CTL_LOWER] != 0;
     if (t < npad && (idlist || stamp == 0 || marks[I.tile_off + it.ty * I.ntx + it.tx] == stamp)) {
         const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
         constexpr int W = kCellT + 4, NR = (W * W + 63) / 64;
@@ -1239,9 +1249,18 @@ __global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(cons
         // a neighbour's sides count when they lie within two cells of this
         // tile: the edge (or corner) of the neighbour facing it
         const int dx = lane % 3 - 1, dy = lane / 3 - 1;
-        const uint32_t need = 1u | (dx < 0 ? 4u : 0u) | (dx > 0 ? 2u : 0u) | (dy < 0 ? 16u : 0u) | (dy > 0 ? 8u : 0u);
-        anyf = tin && (tf & need) == need;
-        any = anyf || idlist;
+        const uint32_t need2 = BAD_ANY | (dx < 0 ? BAD_E2 : 0u) | (dx > 0 ? BAD_W2 : 0u) | (dy < 0 ? BAD_N2 : 0u) |
+                               (dy > 0 ? BAD_S2 : 0u);
+        const uint32_t need4 = BAD_ANY | (dx < 0 ? BAD_E4 : 0u) | (dx > 0 ? BAD_W4 : 0u) | (dy < 0 ? BAD_N4 : 0u) |
+                               (dy > 0 ? BAD_S4 : 0u);
+        anyf = tin && (tf & need2) == need2;
+        // The lowering raises flags only within two cells of an
+        // inadmissible cell X; the next iteration's update and post-detect
+        // lowering reach two cells further.  Whatever this wave reads of
+        // flags being raised, a tile with a cell within four cells of X is
+        // listed, so every tile the lowered cells make active is.
+        const bool any4 = lower && tin && (tf & need4) == need4;
+        any = anyf || any4 || idlist;
 #pragma unroll
         for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
         // The sync of this launch may not have reached the footprint's
@@ -1269,14 +1288,14 @@ __global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(cons
     // an inadmissible cell, which the bad_bits show (no kernel of this
     // launch writes those; where they over-report, a tile listed without
     // work changes nothing).  Otherwise it is the active list.
-    const bool lower = I.ctl[CTL_LOWER] != 0;
     const bool hf = lower ? __ballot(anyf) != 0 : hw;
     // The lowering of the next (inner-loop) iteration, on the tiles of the
     // face list.  Every cell it changes lies within two cells of an
     // inadmissible cell, so its tile is on that list whatever this scan
-    // reads.  Other waves may read flags it has just raised and list a tile
-    // the plain order would not; such a tile holds no flag of its own and
-    // its kernels change nothing.
+    // reads, and every tile within two cells of such a cell is on the
+    // active list (any4 above).  Other waves may read flags it has just
+    // raised and list a tile the plain order would not; such a tile holds
+    // no flag of its own and its kernels change nothing.
     if (hf && lower) {
         const int ci0 = it.tx * kCellT, cj0 = it.ty * kCellT;
         for (int c = lane; c < kCellT * kCellT; c += 64) {

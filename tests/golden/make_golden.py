@@ -9,11 +9,27 @@
    sources by oracle/Makefile, oracle/_ref/libref_exact.so) run on the
    cases of tests/cases.py: the dt sequence, the final U and V, the final
    scheme of each stage of the last step and its MOOD pass counts.
+3. final_<case><n>.npz -- a run through the driver loop of
+   src/mhd_driver.f90:64-95 (cfl_computedt, validTimeStep's clamp to
+   sim_tmax, timeStep_advanceSolution) to its final time: every dt, the
+   sha256 of the final U, V and scheme, the exactly summed (math.fsum)
+   interior totals of each conserved variable and of its modulus, and the
+   final density on every 16th interior cell.  The reference build writes
+   the OT 128^2 and 256^2 ones; OT 1024^2, which the reference build would
+   take hours over on 8 cores, is written by the oracle (hmhd_oracle.c,
+   bit-identical to the reference build on every case compared, including
+   the 256^2 run to t = 0.5 here: `final ot 256 0.5 both`).
 
 Run from the repository root after `make -C oracle ref`:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                 # 1. and 2.
+    python tests/golden/make_golden.py final ot 128 0.5 reference
+    python tests/golden/make_golden.py final ot 256 0.5 both
+    python tests/golden/make_golden.py final ot 1024 0.5 oracle
+The reference keeps its RK stage arrays on the stack: run under
+`ulimit -s unlimited` with OMP_STACKSIZE=512M.
 """
 import hashlib
+import math
 import json
 import os
 import pathlib
@@ -103,7 +119,86 @@ def run_reference(name, n, R, steps):
                 passes=passes, input_sha256=np.array(input_digest(U, V)))
 
 
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def summarize(U, V, S, ngc):
+    inner = U[ngc:-ngc, ngc:-ngc, :]
+    return dict(
+        U_sha256=np.array(sha(U)), V_sha256=np.array(sha(V)), scheme_sha256=np.array(sha(S.astype(np.int32))),
+        totals=np.array([math.fsum(inner[:, :, v].ravel()) for v in range(8)]),
+        abs_totals=np.array([math.fsum(np.abs(inner[:, :, v]).ravel()) for v in range(8)]),
+        rho_sample=np.ascontiguousarray(inner[::16, ::16, 0]))
+
+
+class _Quiet:
+    """The reference prints banners through the Fortran runtime on fd 1."""
+
+    def __enter__(self):
+        self.saved = os.dup(1)
+        self.devnull = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.devnull, 1)
+
+    def __exit__(self, *exc):
+        os.dup2(self.saved, 1)
+        os.close(self.devnull)
+
+
+def drive(step_dt, advance, tmax):
+    """src/mhd_driver.f90:64-95: the first CFL call, then cfl, the clamp of
+    validTimeStep, advance, t = t + dt until t >= tmax."""
+    t, dts = 0.0, []
+    step_dt()
+    while t < tmax:
+        dt = step_dt()
+        if t + dt > tmax:
+            dt = tmax - t
+        advance(dt)
+        t += dt
+        dts.append(dt)
+    return t, dts
+
+
+def run_final(name, n, tmax, engine):
+    cfg, g, U, V = cases.make(name, n=n)
+    out = {}
+    if engine in ("reference", "both"):
+        with _Quiet():
+            ref = O.RefHarness(cfg, "exact")
+            ref.set_state(U, V)
+            t, dts = drive(ref.cfl_dt, ref.advance, tmax)
+            Uf, Vf, S = ref.get_state()
+            ref.flush()
+        out["reference"] = (t, dts, summarize(Uf, Vf, S, cfg.R + 2))
+    if engine in ("oracle", "both"):
+        orc = O.Oracle(cfg)
+        orc.set_state(U, V)
+        t, dts = drive(orc.cfl_dt, orc.advance, tmax)
+        Uf, Vf = orc.get_state()
+        out["oracle"] = (t, dts, summarize(Uf, Vf, orc.stage_schemes()[2], cfg.R + 2))
+    if engine == "both":
+        (t1, d1, s1), (t2, d2, s2) = out["reference"], out["oracle"]
+        same = t1 == t2 and d1 == d2 and all(
+            np.array_equal(s1[k], s2[k]) for k in ("U_sha256", "V_sha256", "scheme_sha256", "totals"))
+        if not same:
+            sys.exit(f"final {name} {n}: oracle and reference differ at t = {tmax}")
+        print(f"final {name} {n}: oracle == reference bit for bit ({len(d1)} steps)")
+    which = "reference" if "reference" in out else "oracle"
+    t, dts, summ = out[which]
+    np.savez_compressed(HERE / f"final_{name}{n}.npz", n=np.array(n), tmax=np.array(tmax), t=np.array(t),
+                        steps=np.array(len(dts)), dts=np.array(dts), input_sha256=np.array(input_digest(U, V)),
+                        engine=np.array(which if engine != "both" else "reference (oracle equal)"), **summ)
+    print(f"final_{name}{n}.npz: {len(dts)} steps to t = {t!r} ({which})")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "final":
+        name, n, tmax, engine = sys.argv[2], int(sys.argv[3]), float(sys.argv[4]), sys.argv[5]
+        if engine != "oracle" and not O.ref_available("exact"):
+            sys.exit("oracle/_ref/libref_exact.so missing: run `make -C oracle ref` first")
+        run_final(name, n, tmax, engine)
+        return
     if REF_PRINT.exists():
         (HERE / "reconstruct_R3_ot128.json").write_text(json.dumps(parse_reference_print(), indent=1) + "\n")
     if not O.ref_available("exact"):
