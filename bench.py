@@ -9,22 +9,31 @@ driver iteration: the CFL reduction (cfl_computedt) and one full SSP-RK3
 step with every MOOD pass (timeStep_advanceSolution).  A zone-update is one
 interior cell advanced one step.
 
-With no --steps the timed region is the whole solve from t = 0 to t = 0.5;
-with --steps K it is exactly K steps from t = 0.  Warm-up steps run on a
-separate copy before the timed region.
+The solve runs from t = 0 to t = 0.5 and is reported whole as
+`full_solve`.  With --steps K the line's value is exactly K of its steps,
+spread evenly over the solve in simulated time (the step starting at the
+first t >= (k + 1/2) * 0.5 / K), each bracketed by stream synchronizations:
+the MOOD loop's depth varies over the solve (1 pass per stage early, about
+8 near t = 0.25, 20.5 on average), so a contiguous window would not stand
+for the configuration.  With no --steps the value is the whole solve.
+Warm-up steps run on a separate copy before the timed region.
 
 One JSON line on stdout.  `roofline` covers the dominant kernel, the face
 kernel in its first-pass form (GP reconstruction + admissibility + HLLC),
-timed with HIP events on the solver's stream inside the timed region.
+timed with HIP events on the solver's stream over the whole solve.
 `cpu_baseline` times the reference's own Fortran step (compiled from its
-sources by oracle/Makefile, OpenMP on the host cores) on a bounded sample of
-the same workload, in a child process started before the GPU is touched.
+sources by oracle/Makefile, OpenMP on the host cores, then one thread) from
+four of the GPU solve's states (t = 0.0625, 0.1875, 0.3125, 0.4375), in a
+child process started before the GPU is touched and idle until the GPU leg
+has finished.
 
-Beside the plain form the line carries two more objects (--forms):
+Beside the plain form the line carries three more objects (--forms):
 `batched`, 256 seeded runs of mixed sizes advanced together in one engine,
-against the same runs launched one after another on their own handles; and
+against the same runs launched one after another on their own handles;
 `served`, Poisson arrivals of mixed requests into a server with an HBM
-budget, with p50/p99 submit-to-result latency and aggregate zone-updates/s.
+budget, with p50/p99 submit-to-result latency and aggregate zone-updates/s;
+and `blast`, the MHD blast at 2048^2 from t = 0 to t = 0.2 (outflow, the
+MOOD cascade down to first order), one solve.
 """
 from __future__ import annotations
 
@@ -88,13 +97,35 @@ def workload(n: int):
     return g, U, V
 
 
-def cpu_baseline_child(n: int, steps: int) -> None:
-    """Runs in a child process: the reference's Fortran step on the host."""
+def _cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_child(n: int, max_steps: int, max_seconds: float) -> None:
+    """Runs in a child process started before the GPU is touched: waits on
+    stdin for a JSON list of saved states (GPU states at sampled steps of
+    the timed solve), then times the reference's Fortran step from each
+    (max_steps steps in all, split evenly, or until max_seconds) with every
+    host core, and one step with one thread from the middle state."""
     sys.path.insert(0, str(ROOT / "oracle"))
+    import ctypes
+    import numpy as np
     import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    g, U, V = workload(n)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     cfg = O.Config(n, n, R=2, ell_over_delta=12.0)
+    line = sys.stdin.readline().strip()
+    paths = json.loads(line) if line else []
+    if not paths:
+        return
+    per = max(1, max_steps // len(paths))
+    ts, steps, el = [], 0, 0.0
+    el1 = None
     if O.ref_available("fast"):
         kind = "reference"
         # the Fortran runtime prints banners on fd 1: keep stdout for the result
@@ -103,26 +134,58 @@ def cpu_baseline_child(n: int, steps: int) -> None:
         os.dup2(devnull, 1)
         try:
             ref = O.RefHarness(cfg, "fast")
-            ref.set_state(U, V)
-            ref.cfl_dt()
-            t0 = time.perf_counter()
-            ref.run(steps, 1.0e30)
-            el = time.perf_counter() - t0
+            for p in paths:
+                st = np.load(p)
+                ts.append(float(st["t"]))
+                ref.set_state(st["U"], st["V"])
+                ref.cfl_dt()
+                t0 = time.perf_counter()
+                for _ in range(per):
+                    ref.advance(ref.cfl_dt())
+                    steps += 1
+                    if time.perf_counter() - t0 > max_seconds / len(paths):
+                        break
+                el += time.perf_counter() - t0
+            # one thread: the OpenMP runtime the reference build links
+            omp = ctypes.CDLL("libomp.so", mode=ctypes.RTLD_GLOBAL)
+            omp.omp_set_num_threads(1)
+            st = np.load(paths[len(paths) // 2])
+            ref.set_state(st["U"], st["V"])
+            dt = ref.cfl_dt()
+            t1 = time.perf_counter()
+            ref.advance(dt)
+            el1 = time.perf_counter() - t1
+            t1_state = float(st["t"])
             ref.flush()
         finally:
             os.dup2(saved, 1)
             os.close(devnull)
+        what = ("reference Fortran (amdflang -O3 -fopenmp; timeStep.f90 serial, as its default(none) "
+                "clause does not compile under OpenMP)")
     else:
         kind = "port"
-        orc = O.Oracle(cfg)
-        orc.set_state(U, V)
-        t0 = time.perf_counter()
-        orc.run(steps, 1.0e30)
-        el = time.perf_counter() - t0
-    print(json.dumps({"value": steps * n * n / el, "unit": UNIT, "cores": threads, "kind": kind,
-                      "sample": f"{steps} steps of the {n}x{n} Orszag-Tang solve from t=0 "
-                                f"({'reference Fortran, amdflang -O3 -fopenmp' if kind == 'reference' else 'C restatement, OpenMP'}), "
-                                f"{el:.2f} s"}))
+        for p in paths:
+            st = np.load(p)
+            ts.append(float(st["t"]))
+            orc = O.Oracle(cfg)
+            orc.set_state(st["U"], st["V"])
+            t0 = time.perf_counter()
+            for _ in range(per):
+                orc.advance(orc.cfl_dt())
+                steps += 1
+            el += time.perf_counter() - t0
+        what = "C restatement (hmhd_oracle.c), OpenMP"
+    print(json.dumps({
+        "value": steps * n * n / el, "unit": UNIT, "cores": threads, "kind": kind,
+        "sample": f"{steps} steps of the {n}x{n} Orszag-Tang solve, {per} from each of the GPU's states at "
+                  f"t = {', '.join(f'{x:.4f}' for x in ts)}, {what}, "
+                  f"{el:.2f} s",
+        "value_1thread": (n * n / el1) if el1 else None,
+        "sample_1thread": f"1 step from the state at t = {t1_state:.4f} with 1 OpenMP thread, {el1:.2f} s"
+        if el1 else None,
+        "cpu_model": _cpu_model(),
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+    }))
 
 
 def _big_stack():
@@ -133,53 +196,36 @@ def _big_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
 
-def run_cpu_baseline(n: int, steps: int):
+def start_cpu_baseline(n: int, steps: int, seconds: float):
+    """The CPU child, started before the GPU is touched; it idles on its
+    stdin until the GPU leg hands it a state (finish_cpu_baseline)."""
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    env.setdefault("OMP_STACKSIZE", "512M")
+    return subprocess.Popen([sys.executable, str(pathlib.Path(__file__).resolve()), "--cpu-child",
+                             "--n", str(n), "--cpu-steps", str(steps), "--cpu-seconds", str(seconds)],
+                            stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                            text=True, env=env, preexec_fn=_big_stack)
+
+
+def finish_cpu_baseline(proc, state_paths):
+    """Hand the child the states (none: nothing to time) and collect its line."""
     try:
-        p = subprocess.run([sys.executable, str(pathlib.Path(__file__).resolve()), "--cpu-child",
-                            "--n", str(n), "--cpu-steps", str(steps)],
-                           capture_output=True, text=True, timeout=600, preexec_fn=_big_stack)
-        line = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
-        if p.returncode != 0 or not line:
-            return {"value": None, "unit": UNIT, "cores": None, "kind": None,
-                    "sample": f"failed: {p.stderr.strip()[-300:]}"}
-        return json.loads(line[-1])
+        out, err = proc.communicate(input=json.dumps([str(p) for p in state_paths]) + "\n", timeout=900)
     except subprocess.TimeoutExpired:
+        proc.kill()
         return {"value": None, "unit": UNIT, "cores": None, "kind": None, "sample": "timed out"}
+    line = [ln for ln in out.splitlines() if ln.startswith("{")]
+    if proc.returncode != 0 or not line:
+        return {"value": None, "unit": UNIT, "cores": None, "kind": None,
+                "sample": f"failed: {err.strip()[-300:]}"}
+    return json.loads(line[-1])
 
 
 # ---------------------------------------------------------------------- #
 # batched and served workloads
-
-BATCH_SIDES = (64, 96, 128, 192, 256, 384)
-T_END_BATCH = {"ot": 0.5, "sedov": 0.05}
-T_END_SERVED = {"ot": 0.05, "blast": 0.01, "sedov": 0.005}
-
-
-def make_run(rt, kind: str, n: int, noise_seed: int | None):
-    """(config, U, V) of one run: the Orszag-Tang vortex above, the MHD blast
-    (rho = 1, p = 10 for r < 0.1 else 0.1, B = (1, 1, 0)/sqrt 2, outflow) or
-    the Sedov blast (E = 1 in the central 3x3 cells, rho = 1, p = 1e-5,
-    B = (0.5, 0, 0), outflow), optionally with 1% seeded uniform noise on
-    rho, v, p and B."""
-    import hosono_2dmhd_amd.ics as ics
-    gamma = 5.0 / 3.0
-    if kind == "ot":
-        g = ics.Grid(n, n, 0.0, 1.0, 0.0, 1.0, R=2)
-        U, V = ics.orszag_tang(g, gamma)
-        bc = rt.BC_PERIODIC
-    elif kind == "blast":
-        g = ics.Grid(n, n, -0.5, 0.5, -0.5, 0.5, R=2)
-        U, V = ics.mhd_blast(g, gamma)
-        bc = rt.BC_OUTFLOW
-    else:
-        g = ics.Grid(n, n, -0.5, 0.5, -0.5, 0.5, R=2)
-        U, V = ics.sedov(g, gamma, energy=1.0, p_bg=1e-5, B=(0.5, 0.0, 0.0))
-        bc = rt.BC_OUTFLOW
-    if noise_seed is not None:
-        U, V = ics.perturb(V, g, gamma, bc, noise_seed, 0.01)
-    cfg = rt.Config(n, n, g.xbeg, g.xend, g.ybeg, g.yend, R=2, ell_over_delta=12.0, gamma=gamma, bc=bc)
-    return cfg, U, V
-
 
 def solo_run(rt, cfg, U, V, tmax: float, nstepmax: int):
     """One run on its own handle, the plain form; returns (seconds, steps)."""
@@ -198,43 +244,39 @@ def solo_run(rt, cfg, U, V, tmax: float, nstepmax: int):
 
 
 def batched_form(rt, args) -> dict:
-    """256 runs (sides from BATCH_SIDES, Orszag-Tang or Sedov, 1% noise, all
-    seeded) advanced together, each for --batch-steps steps or to its end
-    time; then the same runs one after another on their own handles."""
+    """256 runs (hosono_2dmhd_amd.workloads.batched_runs: sides from
+    BATCH_SIDES, Orszag-Tang or Sedov, 1% noise, all seeded) advanced
+    together, each for --batch-steps steps or to its end time; then the
+    same runs one after another on their own handles."""
     import numpy as np
-    rng = np.random.default_rng(args.seed)
-    runs = []
-    for k in range(args.batch_runs):
-        n = int(rng.choice(BATCH_SIDES))
-        kind = "ot" if rng.random() < 0.5 else "sedov"
-        runs.append((kind, n, make_run(rt, kind, n, args.seed * 100003 + k)))
+    import hosono_2dmhd_amd.workloads as wl
+    runs = wl.batched_runs(args.seed, args.batch_runs)
     with rt.Batch() as b:
-        ids = [b.add(cfg, U, V, T_END_BATCH[kind], args.batch_steps) for kind, n, (cfg, U, V) in runs]
+        ids = [b.add(r.cfg, r.U, r.V, r.tmax, args.batch_steps) for r in runs]
         t0 = time.perf_counter()
         b.run(0)
         batch_s = time.perf_counter() - t0
         st = [b.status(i) for i in ids]
-        zu = sum(steps * n * n for (_, steps, _), (_, n, _) in zip(st, runs))
+        zu = sum(steps * r.n * r.n for (_, steps, _), r in zip(st, runs))
         check = [b.download(i) for i in ids[:2]]
     solo_s, solo_zu = 0.0, 0
-    for k, (kind, n, (cfg, U, V)) in enumerate(runs):
-        sec, steps = solo_run(rt, cfg, U, V, T_END_BATCH[kind], args.batch_steps)
+    for r in runs:
+        sec, steps = solo_run(rt, r.cfg, r.U, r.V, r.tmax, args.batch_steps)
         solo_s += sec
-        solo_zu += steps * n * n
+        solo_zu += steps * r.n * r.n
     same = True
     for k in range(len(check)):
-        kind, n, (cfg, U, V) = runs[k]
-        with rt.Solver(cfg) as s:
-            s.upload(U, V)
-            rt.run(s, T_END_BATCH[kind], args.batch_steps)
+        r = runs[k]
+        with rt.Solver(r.cfg) as s:
+            s.upload(r.U, r.V)
+            rt.run(s, r.tmax, args.batch_steps)
             Us, Vs, _ = s.download()
         same = same and bool(np.array_equal(Us, check[k][0]) and np.array_equal(Vs, check[k][1]))
-    sides = [n for _, n, _ in runs]
     return {
-        "workload": f"{len(runs)} runs, sides {list(BATCH_SIDES)} and Orszag-Tang/Sedov drawn with seed "
+        "workload": f"{len(runs)} runs, sides {list(wl.BATCH_SIDES)} and Orszag-Tang/Sedov drawn with seed "
                     f"{args.seed}, 1% seeded noise, {args.batch_steps} steps each (or to t_end "
-                    f"{T_END_BATCH})",
-        "runs": len(runs), "cells": int(sum(n * n for n in sides)), "zone_updates": int(zu),
+                    f"{wl.T_END_BATCH})",
+        "runs": len(runs), "cells": int(sum(r.n * r.n for r in runs)), "zone_updates": int(zu),
         "value": zu / batch_s, "seconds": batch_s,
         "per_instance_plain": {"value": solo_zu / solo_s, "seconds": solo_s,
                                "note": "the same runs one after another, each on its own handle"},
@@ -243,21 +285,42 @@ def batched_form(rt, args) -> dict:
     }
 
 
+def blast_form(rt, args) -> dict:
+    """The MHD blast at 2048^2 (hosono_2dmhd_amd.workloads.make_run:
+    outflow, rho = 1, p = 10 for r < 0.1 else 0.1, B = (1, 1, 0)/sqrt 2),
+    one solve from t = 0 to --blast-tmax, the state resident in HBM."""
+    import hosono_2dmhd_amd.workloads as wl
+    n = args.blast_n
+    cfg, U, V = wl.make_run("blast", n)
+    with rt.Solver(cfg) as s:
+        s.upload(U, V)
+        s.synchronize()
+        t, steps, passes = 0.0, 0, 0
+        t0 = time.perf_counter()
+        while t < args.blast_tmax:
+            dt = rt.valid_time_step(s.cfl_computedt(), t, args.blast_tmax)
+            s.advance_solution(dt)
+            t += dt
+            steps += 1
+            passes += int(s.stats()[0].sum())
+        s.synchronize()
+        el = time.perf_counter() - t0
+    return {"workload": f"mhd_blast_{n}", "nx": n, "ny": n, "t_end": t, "steps": steps, "seconds": el,
+            "value": steps * n * n / el, "unit": UNIT, "ms_per_step": el / steps * 1e3,
+            "mood_passes_per_stage": passes / (3 * steps)}
+
+
 def served_form(rt, args) -> dict:
-    """Poisson arrivals of requests (sides log-uniform in [64, 1024],
-    Orszag-Tang / blast / Sedov, each to its end time in T_END_SERVED) into
-    a server with an HBM budget; latency is submit -> result."""
+    """Poisson arrivals of requests (hosono_2dmhd_amd.workloads.
+    served_requests: sides log-uniform in [64, 1024], Orszag-Tang / blast /
+    Sedov, each to its end time in T_END_SERVED) into a server with an HBM
+    budget; latency is submit -> result."""
     import threading
     import numpy as np
-    rng = np.random.default_rng(args.seed + 1)
-    reqs = []
-    t_arr = 0.0
-    for k in range(args.served_requests):
-        n = int(round(math.exp(rng.uniform(math.log(64.0), math.log(1024.0)))))
-        kind = ("ot", "blast", "sedov")[int(rng.integers(3))]
-        t_arr += float(rng.exponential(1.0 / args.served_rate))
-        cfg, U, V = make_run(rt, kind, n, None)
-        reqs.append((t_arr, kind, n, cfg, U, V))
+    import hosono_2dmhd_amd.workloads as wl
+    reqs = [(r.arrival_s, r.kind, r.n, r.cfg, r.U, r.V)
+            for r in wl.served_requests(args.seed + 1, args.served_requests, args.served_rate)]
+    T_END_SERVED = wl.T_END_SERVED
     budget = int(args.served_budget_gib * (1 << 30))
     ids = [None] * len(reqs)
     with rt.Server(budget) as srv:
@@ -300,8 +363,17 @@ def served_form(rt, args) -> dict:
 
 # ---------------------------------------------------------------------- #
 
-def plain_form(rt, args):
-    """The metric's workload: one Orszag-Tang solve, timed from t = 0."""
+def plain_form(rt, args, cpu_proc=None):
+    """The metric's workload: one Orszag-Tang solve from t = 0 to t_end.
+
+    The whole solve is `full_solve`.  With --steps K the line's value is
+    exactly K of its steps, spread evenly over the solve: the step that
+    starts at the first t >= (k + 1/2) t_end / K, k = 0..K-1, each
+    bracketed by stream synchronizations, so the sample has the solve's mix
+    of quiet and MOOD-heavy steps; without --steps the value is the whole
+    solve.  The GPU states at t = (j + 1/2) t_end / 4, j = 0..3, go to the
+    CPU baseline's child (the downloads are outside every timed region)."""
+    import numpy as np
     g, U, V = workload(args.n)
     cfg = rt.Config(args.n, args.n, R=2, ell_over_delta=12.0)
 
@@ -312,51 +384,94 @@ def plain_form(rt, args):
             w.advance_solution(w.cfl_computedt())
         w.synchronize()
 
-    # timed region: HIP events bracket only the first-pass face kernel
+    # HIP events bracket only the first-pass face kernel, except in the
+    # breakdown window, where every kernel family is timed (but the timed
+    # sample's steps)
     s = rt.Solver(cfg)
     s.upload(U, V)
     s.profile(["face_full"])
     s.synchronize()
     s.profile_read(reset=True)
-    tmax = args.tmax if args.steps == 0 else float("inf")
-    limit = args.steps if args.steps > 0 else 10_000_000
-    t = 0.0
-    steps = 0
-    passes = 0
-    # every kernel family is timed over a window of steps from the middle
-    # of the solve (shocks formed, many MOOD passes); events around every
-    # launch slow those steps slightly
-    win0 = args.breakdown_at if args.steps == 0 else max(0, limit // 2 - args.breakdown_steps // 2)
-    win1 = win0 + args.breakdown_steps
-    snap0 = snap1 = None
-    w_t0 = w_t1 = None
-    t0 = time.perf_counter()
-    while t < tmax and steps < limit:
-        if steps == win0:
+    tmax, K, zones = args.tmax, args.steps, args.n * args.n
+    samp_t = [(k + 0.5) * tmax / K for k in range(K)] if K > 0 else []
+    cpu_t = [(j + 0.5) * tmax / 4 for j in range(4)] if cpu_proc is not None else []
+    si = ci = 0
+    state_paths = []
+    samp_s, samp_n, samp_passes, samp_first = 0.0, 0, 0, []
+    t, steps, passes = 0.0, 0, 0
+    bd = None           # [first step, profile snapshot, wall start] of the breakdown window
+    breakdown, bd_range, bd_wall = {}, None, None
+    bd_first = None if K > 0 else args.breakdown_at
+    full_s = 0.0
+    seg0 = time.perf_counter()
+    while t < tmax:
+        sample = si < len(samp_t) and t >= samp_t[si]
+        save = ci < len(cpu_t) and t >= cpu_t[ci]
+        if sample or save:
+            s.synchronize()
+            full_s += time.perf_counter() - seg0
+            while ci < len(cpu_t) and t >= cpu_t[ci]:
+                if ci == 0 or state_paths[-1][1] != steps:
+                    Um, Vm, _ = s.download(want_scheme=False)
+                    path = pathlib.Path(f"/tmp/hmhd_bench_state_{os.getpid()}_{ci}.npz")
+                    np.savez(path, U=Um, V=Vm, t=np.array(t))
+                    del Um, Vm
+                    state_paths.append((path, steps))
+                ci += 1
+            s.synchronize()
+            seg0 = time.perf_counter()
+        if bd is None and bd_first is not None and steps == bd_first and args.breakdown_steps > 0:
             s.profile(True)
-            snap0 = s.profile_read()
-            w_t0 = time.perf_counter()
-        elif steps == win1:
+            bd = [steps, s.profile_read(), time.perf_counter()]
+        elif bd is not None and bd_range is None and steps == bd[0] + args.breakdown_steps:
             snap1 = s.profile_read()
-            w_t1 = time.perf_counter()
+            bd_wall = time.perf_counter() - bd[2]
             s.profile(["face_full"])
+            breakdown = {k: (snap1[k][0] - bd[1][k][0], snap1[k][1] - bd[1][k][1]) for k in snap1}
+            bd_range = [bd[0], steps]
+        in_bd = bd is not None and bd_range is None
+        if sample and in_bd:
+            s.profile(["face_full"])
+        t_step = time.perf_counter()
         dt = rt.valid_time_step(s.cfl_computedt(), t, tmax)
         s.advance_solution(dt)
+        p = int(s.stats()[0].sum())
+        if sample:
+            s.synchronize()
+            samp_s += time.perf_counter() - t_step
+            samp_n += 1
+            samp_passes += p
+            samp_first.append(steps)
+            if in_bd:
+                s.profile(True)
+            while si < len(samp_t) and t >= samp_t[si]:
+                si += 1
+            if si >= len(samp_t) // 2 + 1 and bd_first is None:
+                bd_first = steps + 1  # the breakdown window starts after the middle sample
         t += dt
         steps += 1
-        passes += int(s.stats()[0].sum())
+        passes += p
     s.synchronize()
-    elapsed = time.perf_counter() - t0
+    full_s += time.perf_counter() - seg0
     prof = s.profile_read()
     s.close()
-    if snap1 is None:
-        snap1 = prof
-        win1 = steps
-        w_t1 = time.perf_counter()
-    breakdown = {k: (snap1[k][0] - snap0[k][0], snap1[k][1] - snap0[k][1]) for k in snap1} if snap0 else {}
+    cpu = finish_cpu_baseline(cpu_proc, [q for q, _ in state_paths]) if cpu_proc is not None else None
+    for q, _ in state_paths:
+        q.unlink(missing_ok=True)
 
-    zones = args.n * args.n
-    value = steps * zones / elapsed
+    full = {"value": steps * zones / full_s, "unit": UNIT, "steps": steps, "seconds": full_s, "t_end": t,
+            "ms_per_step": full_s / steps * 1e3, "mood_passes_per_stage": passes / (3 * steps),
+            "note": "the whole solve from t=0 in the same run; it contains the sampled steps, whose "
+                    "synchronizations cost it a little, and the breakdown window, whose events do"}
+    if K > 0 and samp_n > 0:
+        value, n_timed, sec = samp_n * zones / samp_s, samp_n, samp_s
+        sampled = {"steps": samp_first, "passes_per_stage": samp_passes / (3 * samp_n),
+                   "rule": f"the step starting at the first t >= (k + 1/2) * {tmax} / {K}, k = 0..{K - 1}, "
+                           f"each bracketed by stream synchronizations"}
+    else:
+        value, n_timed, sec = full["value"], steps, full_s
+        sampled = {"steps": "all", "passes_per_stage": full["mood_passes_per_stage"], "rule": "the whole solve"}
+
     ms_full, n_full = prof["face_full"]
     avg_s = ms_full / n_full / 1e3 if n_full else float("nan")
     flops_launch = zones * FLOP_FACE_KERNEL_R2
@@ -366,29 +481,34 @@ def plain_form(rt, args):
     if tf.exists() and args.n == 1024:
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
     roofline = {
-        "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
         "kernel": "k_face<2,FULL>", "flops_per_launch": flops_launch, "avg_launch_ms": avg_s * 1e3,
-        "launches": n_full, "peak_note": "FP64 vector peak (equal to the FP64 MFMA peak on MI355X)",
+        "launches": n_full,
+        "peak_note": "FP64 vector (VALU) peak; the kernel uses no MFMA (the FP64 matrix peak is the same on MI355X)",
         "hbm_achieved_gbs": zones * BYTES_FACE_KERNEL / avg_s / 1e9, "hbm_peak_gbs": PEAK_HBM_GBS,
     }
     mood_iters, tiles = breakdown.pop("tiles", (0, 0))
     line = {
-        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": steps,
-        "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": n_timed,
+        "warmup": args.warmup, "ms_per_step": sec / n_timed * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"orszag_tang_{args.n}", "nx": args.n, "ny": args.n, "gp_radius": 2,
-                   "ell_over_delta": 12.0, "riemann": "hllc", "cfl": 0.8, "t_end": t,
-                   "form": "plain", "mood_passes_per_stage": passes / (3 * steps)},
+                   "ell_over_delta": 12.0, "riemann": "hllc", "cfl": 0.8, "t_end": tmax,
+                   "form": "plain", "timed_steps": sampled["rule"],
+                   "mood_passes_per_stage": sampled["passes_per_stage"]},
+        "timed_sample_steps": sampled["steps"],
+        "full_solve": full,
         "roofline": roofline,
         "flops_per_zone_update": FLOP_ZONE_UPDATE_R2,
-        "kernel_breakdown": {"steps": [win0, win1],
+        "kernel_breakdown": {"steps": bd_range,
                              "ms": {k: v[0] for k, v in breakdown.items()},
                              "launches": {k: v[1] for k, v in breakdown.items()},
                              "mood_iterations": int(mood_iters), "active_tiles_visited": int(tiles),
                              # host wall time of the window's steps, to set against the
                              # kernel families' event times (the gap is GPU idle time)
-                             "window_wall_ms": (w_t1 - w_t0) * 1e3 if w_t0 is not None else None},
+                             "window_wall_ms": bd_wall * 1e3 if bd_wall is not None else None},
+        "cpu_baseline": cpu,
     }
     return line
 
@@ -396,16 +516,21 @@ def plain_form(rt, args):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the whole solve to t=0.5)")
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps, spread evenly over the solve (0: the whole solve)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--tmax", type=float, default=0.5)
-    ap.add_argument("--breakdown-at", type=int, default=700, help="first step of the all-kernel timing window")
+    ap.add_argument("--breakdown-at", type=int, default=700,
+                    help="first step of the all-kernel timing window when --steps is 0 (else after the middle sample)")
     ap.add_argument("--breakdown-steps", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="steps in the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="steps in the CPU sample (split over its four states)")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0, help="cap on the CPU sample's seconds")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
-    ap.add_argument("--forms", default="plain,batched,served",
-                    help="comma list of plain, batched, served (the line's value is always the plain form's)")
+    ap.add_argument("--forms", default="plain,batched,served,blast",
+                    help="comma list of plain, batched, served, blast (the line's value is always the plain form's)")
+    ap.add_argument("--blast-n", type=int, default=2048)
+    ap.add_argument("--blast-tmax", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -417,7 +542,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.cpu_child:
-        cpu_baseline_child(args.n, args.cpu_steps)
+        cpu_baseline_child(args.n, args.cpu_steps, args.cpu_seconds)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this one-GPU build "
@@ -426,18 +551,23 @@ def main() -> int:
         return 2
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
 
-    cpu = None if args.no_cpu else run_cpu_baseline(args.n, args.cpu_steps)
+    # the CPU child starts before anything touches the GPU; it waits for a state
+    cpu = None if args.no_cpu else start_cpu_baseline(args.n, args.cpu_steps, args.cpu_seconds)
 
     load_package()
     import hosono_2dmhd_amd.runtime as rt
-    line = plain_form(rt, args)
-    line["cpu_baseline"] = cpu
+    line = plain_form(rt, args, cpu)
+    print(f"plain: {line['value']:.4g} zone-updates/s; full solve {line['full_solve']['value']:.4g}",
+          file=sys.stderr, flush=True)
     if "batched" in forms:
         line["batched"] = batched_form(rt, args)
         print(f"batched: {line['batched']['value']:.4g} zone-updates/s", file=sys.stderr, flush=True)
     if "served" in forms:
         line["served"] = served_form(rt, args)
         print(f"served: p50 {line['served']['p50_latency_s']:.3f} s", file=sys.stderr, flush=True)
+    if "blast" in forms:
+        line["blast"] = blast_form(rt, args)
+        print(f"blast: {line['blast']['value']:.4g} zone-updates/s", file=sys.stderr, flush=True)
     print(json.dumps(line))
     return 0
 

@@ -19,12 +19,18 @@
    take hours over on 8 cores, is written by the oracle (hmhd_oracle.c,
    bit-identical to the reference build on every case compared, including
    the 256^2 run to t = 0.5 here: `final ot 256 0.5 both`).
+4. batched_seed<s>.json / served_seed<s>.json -- the first runs of the
+   batched and served workloads (hosono_2dmhd_amd.workloads, as bench.py
+   draws them) through the oracle's driver loop, each to its end time or a
+   step cap: every dt and the sha256 of the final U and V.
 
 Run from the repository root after `make -C oracle ref`:
     python tests/golden/make_golden.py                 # 1. and 2.
     python tests/golden/make_golden.py final ot 128 0.5 reference
     python tests/golden/make_golden.py final ot 256 0.5 both
-    python tests/golden/make_golden.py final ot 1024 0.5 oracle
+    python tests/golden/make_golden.py final ot 1024 0.5 oracle [checkpoint.npz]
+    python tests/golden/make_golden.py batched 2024 32 20   # 4.
+    python tests/golden/make_golden.py served 2025 12 5     # 4.
 The reference keeps its RK stage arrays on the stack: run under
 `ulimit -s unlimited` with OMP_STACKSIZE=512M.
 """
@@ -145,11 +151,18 @@ class _Quiet:
         os.close(self.devnull)
 
 
-def drive(step_dt, advance, tmax):
+def drive(step_dt, advance, tmax, ckpt=None):
     """src/mhd_driver.f90:64-95: the first CFL call, then cfl, the clamp of
-    validTimeStep, advance, t = t + dt until t >= tmax."""
+    validTimeStep, advance, t = t + dt until t >= tmax.  ckpt: optional
+    (load, save) pair resuming from and saving (t, dts) with the state
+    every 50 steps (a step from a stored U, V is the step the run would
+    have taken)."""
     t, dts = 0.0, []
-    step_dt()
+    resumed = ckpt[0]() if ckpt else None
+    if resumed is not None:
+        t, dts = resumed
+    else:
+        step_dt()
     while t < tmax:
         dt = step_dt()
         if t + dt > tmax:
@@ -157,10 +170,12 @@ def drive(step_dt, advance, tmax):
         advance(dt)
         t += dt
         dts.append(dt)
+        if ckpt and len(dts) % 50 == 0:
+            ckpt[1](t, dts)
     return t, dts
 
 
-def run_final(name, n, tmax, engine):
+def run_final(name, n, tmax, engine, ckpt_path=None):
     cfg, g, U, V = cases.make(name, n=n)
     out = {}
     if engine in ("reference", "both"):
@@ -174,7 +189,27 @@ def run_final(name, n, tmax, engine):
     if engine in ("oracle", "both"):
         orc = O.Oracle(cfg)
         orc.set_state(U, V)
-        t, dts = drive(orc.cfl_dt, orc.advance, tmax)
+        ckpt = None
+        if ckpt_path:
+            path = pathlib.Path(ckpt_path)
+
+            def load():
+                if not path.exists():
+                    return None
+                c = np.load(path)
+                orc.set_state(c["U"], c["V"])
+                print(f"resumed at step {len(c['dts'])}, t = {float(c['t'])!r}", flush=True)
+                return float(c["t"]), [float(x) for x in c["dts"]]
+
+            def save(t, dts):
+                Uc, Vc = orc.get_state()
+                tmp = path.with_suffix(".tmp.npz")
+                np.savez(tmp, U=Uc, V=Vc, t=np.array(t), dts=np.array(dts))
+                tmp.replace(path)
+                print(f"step {len(dts)} t = {t:.6f}", flush=True)
+
+            ckpt = (load, save)
+        t, dts = drive(orc.cfl_dt, orc.advance, tmax, ckpt)
         Uf, Vf = orc.get_state()
         out["oracle"] = (t, dts, summarize(Uf, Vf, orc.stage_schemes()[2], cfg.R + 2))
     if engine == "both":
@@ -192,12 +227,51 @@ def run_final(name, n, tmax, engine):
     print(f"final_{name}{n}.npz: {len(dts)} steps to t = {t!r} ({which})")
 
 
+def oracle_config(c):
+    """The oracle's Config of a runtime Config."""
+    return O.Config(c.nx, c.ny, c.xbeg, c.xend, c.ybeg, c.yend, c.R, c.ell_over_delta, c.gamma, c.small_p,
+                    c.small_rho, c.small_B, c.force_hydro, c.bc, c.cfl)
+
+
+def run_mix(kind, seed, count, nstepmax):
+    """Fixtures of the batched and served workloads (hosono_2dmhd_amd.
+    workloads): each run through the oracle's driver loop to its end time or
+    nstepmax steps; JSON with the dts and the sha256 of the final U and V."""
+    import hosono_2dmhd_amd.workloads as wl
+    runs = wl.batched_runs(seed, count) if kind == "batched" else wl.served_requests(seed, count, 12.0)
+    out = []
+    for r in runs:
+        orc = O.Oracle(oracle_config(r.cfg))
+        orc.set_state(r.U, r.V)
+        t, dts = 0.0, []
+        orc.cfl_dt()
+        while t < r.tmax and len(dts) < nstepmax:
+            dt = orc.cfl_dt()
+            if t + dt > r.tmax:
+                dt = r.tmax - t
+            orc.advance(dt)
+            t += dt
+            dts.append(dt)
+        Uf, Vf = orc.get_state()
+        out.append({"kind": r.kind, "n": r.n, "noise_seed": r.noise_seed, "tmax": r.tmax,
+                    "input_sha256": input_digest(r.U, r.V), "steps": len(dts), "t": t, "dts": dts,
+                    "U_sha256": sha(Uf), "V_sha256": sha(Vf)})
+        print(f"{kind} run {len(out) - 1}: {r.kind} {r.n}^2, {len(dts)} steps", flush=True)
+    path = HERE / f"{kind}_seed{seed}.json"
+    path.write_text(json.dumps({"workload": kind, "seed": seed, "count": count, "nstepmax": nstepmax,
+                                "engine": "oracle (hmhd_oracle.c)", "runs": out}, indent=1) + "\n")
+    print(f"wrote {path.name}")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] in ("batched", "served"):
+        run_mix(sys.argv[1], int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "final":
         name, n, tmax, engine = sys.argv[2], int(sys.argv[3]), float(sys.argv[4]), sys.argv[5]
         if engine != "oracle" and not O.ref_available("exact"):
             sys.exit("oracle/_ref/libref_exact.so missing: run `make -C oracle ref` first")
-        run_final(name, n, tmax, engine)
+        run_final(name, n, tmax, engine, sys.argv[6] if len(sys.argv) > 6 else None)
         return
     if REF_PRINT.exists():
         (HERE / "reconstruct_R3_ot128.json").write_text(json.dumps(parse_reference_print(), indent=1) + "\n")
