@@ -89,6 +89,10 @@ int profile_read(hmhd::Engine& e, double* ms, int64_t* launches, int32_t reset) 
 }
 }  // namespace
 
+namespace hmhd {
+void set_free_error(const char* msg) { g_create_err = msg; }
+}  // namespace hmhd
+
 // ------------------------------------------------------------------ //
 // the request loop of the served form
 

@@ -1376,21 +1376,8 @@ __global__ __launch_bounds__(256) void k_cfl(const Inst* __restrict__ insts, con
             b2 = V.by;
             b3 = V.bz;
         }
-        double cs;
-        if (I.hydro) {
-            cs = __builtin_sqrt(I.gamma * p / d);
-        } else {
-            const double BB = b1 * b1 + b2 * b2 + b3 * b3;
-            double ab[3] = {__builtin_fabs(b1), __builtin_fabs(b2), __builtin_fabs(b3)};
-            double bn = ab[0];
-            for (int k = 1; k < 3; k++)
-                if (__builtin_isnan(bn) || ab[k] < bn) bn = ab[k];
-            cs = eigen_cfs(I.gamma, d, p, BB, bn);
-        }
-        const double xl = __builtin_fabs(vx) + cs, yl = __builtin_fabs(vy) + cs;
-        bx = __double_as_longlong(xl);
-        by = __double_as_longlong(yl);
         // negative zero cannot occur (|v| + c >= 0); NaN sorts above +inf
+        cfl_speed_bits(I.gamma, I.hydro, d, vx, vy, p, b1, b2, b3, bx, by);
     }
     bx = wave_max_u64(bx);
     by = wave_max_u64(by);

@@ -86,6 +86,30 @@ __device__ __forceinline__ double eigen_cfs(double gam, double rho, double p, do
     return __builtin_sqrt(fmaxF(1.0e-14, cfs));
 }
 
+// The two speeds cfl_computedt maximises for one cell (src/cfl.f90:28-48):
+// |vx| + c and |vy| + c, c the sound speed (hydro) or the fast speed with
+// B_N = MIN(|Bx|, |By|, |Bz|), NaN-skipping as MINVAL; returned as the bit
+// patterns of the two non-negative values, whose unsigned order is their
+// numeric order (NaN above +inf).
+__device__ __forceinline__ void cfl_speed_bits(double gamma, int hydro, double d, double vx, double vy, double p,
+                                               double b1, double b2, double b3, unsigned long long& bx,
+                                               unsigned long long& by) {
+    double cs;
+    if (hydro) {
+        cs = __builtin_sqrt(gamma * p / d);
+    } else {
+        const double BB = b1 * b1 + b2 * b2 + b3 * b3;
+        const double ab[3] = {__builtin_fabs(b1), __builtin_fabs(b2), __builtin_fabs(b3)};
+        double bn = ab[0];
+        for (int k = 1; k < 3; k++)
+            if (__builtin_isnan(bn) || ab[k] < bn) bn = ab[k];
+        cs = eigen_cfs(gamma, d, p, BB, bn);
+    }
+    const double xl = __builtin_fabs(vx) + cs, yl = __builtin_fabs(vy) + cs;
+    bx = (unsigned long long)__double_as_longlong(xl);
+    by = (unsigned long long)__double_as_longlong(yl);
+}
+
 // fast speed of the split system in direction DIR (0 = x, 1 = y)
 template <int DIR>
 __device__ __forceinline__ double fast_speed(const Phys& ph, const Prim& V) {

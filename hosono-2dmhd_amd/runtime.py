@@ -38,7 +38,7 @@ EXPORTS = (
     "hmhd_set_gp",
     "hmhd_upload_state", "hmhd_cfl_dt", "hmhd_advance", "hmhd_download", "hmhd_stats",
     "hmhd_record_stage_schemes", "hmhd_stage_schemes", "hmhd_profile", "hmhd_profile_read",
-    "hmhd_synchronize",
+    "hmhd_synchronize", "hmhd_cfl_dt_noalloc",
     "hmhd_batch_create", "hmhd_batch_destroy", "hmhd_batch_last_error", "hmhd_batch_add", "hmhd_batch_run",
     "hmhd_batch_status", "hmhd_batch_dims", "hmhd_batch_download", "hmhd_batch_profile",
     "hmhd_batch_profile_read",
@@ -133,6 +133,7 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.hmhd_profile.argtypes = [vp, C.c_int32]
     lib.hmhd_profile_read.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int32]
     lib.hmhd_synchronize.argtypes = [vp]
+    lib.hmhd_cfl_dt_noalloc.argtypes = [C.POINTER(_Config), vp, vp, vp, vp]
     i64, i64p = C.c_int64, C.POINTER(C.c_int64)
     lib.hmhd_batch_create.argtypes = [C.POINTER(vp)]
     lib.hmhd_batch_destroy.argtypes = [vp]
@@ -301,6 +302,19 @@ class Solver:
 
     def synchronize(self):
         self._check(self.lib.hmhd_synchronize(self.h))
+
+
+def cfl_dt_noalloc(cfg: Config, U_ptr: int, V_ptr: int | None, dt_ptr: int, stream: int | None = None) -> None:
+    """hmhd_cfl_dt_noalloc: enqueue the CFL dt of a state in device memory
+    (U(8,mx,my) at U_ptr, V(10,mx,my) at V_ptr or None) into the device
+    double at dt_ptr, on `stream` (a hipStream_t as an int; None: the null
+    stream), allocating nothing."""
+    lib = load_library()
+    c = cfg.to_c()
+    rc = lib.hmhd_cfl_dt_noalloc(C.byref(c), C.c_void_p(U_ptr), C.c_void_p(V_ptr) if V_ptr else None,
+                                 C.c_void_p(dt_ptr), C.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise HmhdError(rc, lib.hmhd_last_error(None).decode())
 
 
 def _padded(cfg: Config):

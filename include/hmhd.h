@@ -132,6 +132,22 @@ int hmhd_profile_read(hmhd_t* h, double* ms, int64_t* launches, int32_t reset);
 /* Wait for all work enqueued on the handle. */
 int hmhd_synchronize(hmhd_t* h);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the dt of hmhd_cfl_dt (the same
+ * arithmetic and bits, src/cfl.f90:14-52) for a state the caller keeps in
+ * device memory, without a handle and without allocating.  U is
+ * U(8, mx, my) and V, optional, V(10, mx, my) in the reference's layouts
+ * (read as the first step's cfl_computedt reads the IC's primitives; NULL:
+ * derived from U); dt is one double of device memory, written by the
+ * last of two kernels enqueued on `stream` (a hipStream_t; NULL for the
+ * null stream).  The block maxima go to a fixed 16 KiB device array per
+ * GPU: calls that overlap on one GPU race.  At most HMHD_NOALLOC_MAX_CELLS
+ * padded cells; larger grids, and errors, return a code with the message
+ * in hmhd_last_error(NULL). */
+#define HMHD_NOALLOC_MAX_CELLS (1LL << 30)
+int hmhd_cfl_dt_noalloc(const hmhd_config* cfg, const double* U, const double* V, double* dt, void* stream);
+
 /* ------------------------------------------------------------------ *
  * Batch: many independent runs of the driver loop of src/mhd_driver.f90:
  * 76-95 (cfl_computedt, validTimeStep, timeStep_advanceSolution until
