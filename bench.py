@@ -337,6 +337,13 @@ def served_form(rt, args) -> dict:
         res = [srv.wait(i, timeout_s=600.0, arrays=False) for i in ids]
         stats = srv.stats()
     lat = np.array([r.latency_s for r in res])
+    sides = np.array([n for _, _, n, _, _, _ in reqs])
+    classes = {}
+    for name, lo, hi in (("small (<256^2)", 0, 256), ("medium (256^2..511^2)", 256, 512), ("large (>=512^2)", 512, 1 << 30)):
+        sel = (sides >= lo) & (sides < hi)
+        if sel.any():
+            classes[name] = {"requests": int(sel.sum()), "p50_latency_s": float(np.percentile(lat[sel], 50)),
+                             "p99_latency_s": float(np.percentile(lat[sel], 99))}
     zu = sum(r.zone_updates for r in res)
     span = max(r.done_s for r in res) - min(r.submit_s for r in res)
     solo_s, solo_zu = 0.0, 0
@@ -350,9 +357,10 @@ def served_form(rt, args) -> dict:
                     f"Orszag-Tang/blast/Sedov to t_end {T_END_SERVED}, seed {args.seed + 1}",
         "hbm_budget_bytes": budget, "completed": int(sum(r.state == rt.REQ_DONE for r in res)),
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-        "mean_latency_s": float(lat.mean()), "zone_updates": int(zu), "value": zu / span, "seconds": span,
+        "mean_latency_s": float(lat.mean()), "latency_by_size": classes,
+        "zone_updates": int(zu), "value": zu / span, "seconds": span,
         "peak_active": stats["peak_active"], "peak_bytes": stats["peak_bytes"],
-        "engine_steps": stats["engine_steps"], "busy_fraction": stats["busy_s"] / span,
+        "engine_steps": stats["engine_steps"], "busy_lane_seconds_per_second": stats["busy_s"] / span,
     }
     if args.served_plain:
         out["per_instance_plain"] = {"value": solo_zu / solo_s, "seconds": solo_s,

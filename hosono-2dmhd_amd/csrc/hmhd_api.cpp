@@ -6,6 +6,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -96,7 +97,18 @@ void set_free_error(const char* msg) { g_create_err = msg; }
 // ------------------------------------------------------------------ //
 // the request loop of the served form
 
+// The server runs its admitted solves in lanes: a lane is an engine with its
+// own stream and worker thread, and lanes overlap on the device (their
+// engines share no stream and make no device-wide synchronization).  Lane
+// 0 batches any number of solves, stepped together; each further lane
+// holds one large solve at a time, which then steps alone with the
+// single-solve kernels instead of waiting in lockstep for the batch.  A
+// request of at least kSoloCells interior cells takes a free solo lane if
+// there is one, otherwise it joins the batch.
 struct hmhd_server_handle {
+    static constexpr int kSoloLanes = 2;
+    static constexpr long long kSoloCells = 512 * 512;
+
     struct Req {
         int64_t id = 0;
         hmhd_config cfg{};
@@ -104,26 +116,36 @@ struct hmhd_server_handle {
         double tmax = 0.0;
         int32_t nstepmax = 0;
         size_t bytes = 0;
-        int64_t solve = -1;
+        int lane = -1;
+        int64_t gp_wait = -1;  // re-queued for a GP table set: wait until more than this many solves finished
         hmhd_result res{};
         std::vector<double> Uout, Vout;
         std::vector<int32_t> Sout;
         bool done = false;
     };
 
-    hmhd::Engine eng;  // touched by the worker thread only
-    hmhd::Runner run{eng};
+    struct Lane {
+        hmhd::Engine eng;  // touched by this lane's thread only
+        hmhd::Runner run{eng};
+        bool solo = false;
+        std::vector<int64_t> todo;  // assigned requests not yet added (under mu)
+        int assigned = 0;           // assigned and not finished (under mu)
+        std::unordered_map<int64_t, int64_t> by_solve;  // solve id -> request id (this lane's thread)
+        std::condition_variable cv;
+        std::thread th;
+    };
+
     uint64_t budget = 0;
     int32_t max_active = 0;
-    std::mutex mu;  // guards everything below
-    std::condition_variable cv_work, cv_done;
+    std::mutex mu;  // guards the queue, the requests' states, the lanes' todo/assigned, the stats
+    std::condition_variable cv_done;
     std::deque<int64_t> queue;
-    std::unordered_map<int64_t, Req> reqs;
-    std::unordered_map<int64_t, int64_t> by_solve;  // solve id -> request id
+    std::unordered_map<int64_t, Req> reqs;  // elements keep their addresses across inserts
+    std::vector<std::unique_ptr<Lane>> lanes;
+    uint64_t reserved = 0;  // footprint of the assigned requests
     int64_t next_req = 1;
     bool stop = false;
     std::string err;
-    std::thread worker;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     hmhd_server_stats st{};
     double busy_s = 0.0;
@@ -136,11 +158,14 @@ struct hmhd_server_handle {
         r.done = true;
     }
 
-    // Admit queued requests in arrival order while they fit the budget and
-    // the slot limit; a request that does not fit yet is passed over for
-    // later, smaller ones.  One larger than the whole budget fails at once.
+    // Assign queued requests to lanes in arrival order while they fit the
+    // budget and the slot limit; a request that does not fit yet is passed
+    // over for later, smaller ones.  One larger than the whole budget fails
+    // at once.
     void admit_locked() {
         bool changed = false;
+        int active = 0;
+        for (auto& l : lanes) active += l->assigned;
         for (auto it = queue.begin(); it != queue.end();) {
             Req& r = reqs[*it];
             if (r.bytes > budget) {
@@ -150,96 +175,152 @@ struct hmhd_server_handle {
                 changed = true;
                 continue;
             }
-            const bool slot = max_active <= 0 || (int32_t)by_solve.size() < max_active;
-            if (!slot) break;
-            if (eng.bytes_in_use() + r.bytes > budget) {
+            if (max_active > 0 && active >= max_active) break;
+            if (reserved + r.bytes > budget || r.gp_wait == st.completed) {
                 ++it;
                 continue;
             }
-            const int64_t sid = eng.add(r.cfg);
-            if (sid < 0 || eng.upload(sid, r.U.data(), r.V.empty() ? nullptr : r.V.data()) != HMHD_OK) {
-                err = eng.err;
-                if (sid >= 0) (void)eng.remove(sid);
-                finish_locked(r, HMHD_REQ_FAILED);
-                it = queue.erase(it);
-                changed = true;
-                continue;
-            }
-            r.solve = sid;
+            int lane = 0;
+            if ((long long)r.cfg.nx * r.cfg.ny >= kSoloCells)
+                for (size_t k = 1; k < lanes.size(); k++)
+                    if (lanes[k]->assigned == 0) {
+                        lane = (int)k;
+                        break;
+                    }
+            Lane& L = *lanes[lane];
+            r.lane = lane;
             r.res.state = HMHD_REQ_RUNNING;
             r.res.admit_s = now();
-            std::vector<double>().swap(r.U);
-            std::vector<double>().swap(r.V);
-            run.track(sid, r.tmax, r.nstepmax);
-            by_solve[sid] = r.id;
+            L.todo.push_back(r.id);
+            L.assigned += 1;
+            active += 1;
+            reserved += r.bytes;
+            L.cv.notify_one();
             it = queue.erase(it);
         }
-        st.active = (int32_t)by_solve.size();
+        st.active = active;
         st.queued = (int32_t)queue.size();
         if (st.active > st.peak_active) st.peak_active = st.active;
-        st.bytes_in_use = eng.bytes_in_use();
+        st.bytes_in_use = reserved;
         if (st.bytes_in_use > st.peak_bytes) st.peak_bytes = st.bytes_in_use;
         if (changed) cv_done.notify_all();
     }
 
-    void loop() {
-        std::vector<int64_t> fin;
+    // a request leaves its lane: its reservation and slot are freed
+    void release_locked(Lane& L, Req& r) {
+        L.assigned -= 1;
+        reserved -= r.bytes;
+    }
+
+    void lane_loop(Lane& L) {
+        std::vector<int64_t> todo, fin;
         for (;;) {
             {
                 std::unique_lock<std::mutex> lk(mu);
-                admit_locked();
-                // with nothing admitted every queued request fits an empty
-                // engine or has failed, so this wait cannot spin
-                cv_work.wait(lk, [&] { return stop || !by_solve.empty() || !queue.empty(); });
+                L.cv.wait(lk, [&] { return stop || !L.todo.empty() || !L.by_solve.empty(); });
                 if (stop) return;
-                admit_locked();
-                if (by_solve.empty()) continue;
+                todo.swap(L.todo);
             }
-            // one engine step of every admitted run, outside the lock so
-            // submissions and polls go on meanwhile
+            // new solves: GP tables, device memory and upload, outside the lock
+            for (int64_t rid : todo) {
+                Req* r;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    r = &reqs[rid];
+                }
+                const int64_t sid = L.eng.add(r->cfg);
+                const bool gp_full = sid < 0 && L.eng.gp_full;
+                int rc = sid < 0 ? HMHD_ERR_DEVICE : L.eng.upload(sid, r->U.data(), r->V.empty() ? nullptr : r->V.data());
+                if (rc != HMHD_OK) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (sid >= 0) (void)L.eng.remove(sid);
+                    release_locked(L, *r);
+                    bool others = false;
+                    for (auto& l : lanes) others = others || l->assigned > 0;
+                    if (gp_full && others) {
+                        // every GP table set is in use by running solves: wait for one to finish
+                        r->lane = -1;
+                        r->gp_wait = st.completed;
+                        r->res.state = HMHD_REQ_QUEUED;
+                        queue.push_front(rid);
+                    } else {
+                        err = L.eng.err;
+                        finish_locked(*r, HMHD_REQ_FAILED);
+                        cv_done.notify_all();
+                    }
+                    admit_locked();
+                    continue;
+                }
+                std::vector<double>().swap(r->U);
+                std::vector<double>().swap(r->V);
+                L.run.track(sid, r->tmax, r->nstepmax);
+                L.by_solve[sid] = rid;
+            }
+            todo.clear();
+            if (L.by_solve.empty()) continue;
+            // one engine step of every solve of the lane, outside the lock
             fin.clear();
             const double a = now();
-            const int rc = run.step(fin);
+            const int rc = L.run.step(fin);
             const double b = now();
-            std::lock_guard<std::mutex> lk(mu);
-            busy_s += b - a;
-            st.engine_steps += 1;
             if (rc != HMHD_OK) {
-                // a device error leaves every running request undefined
-                err = eng.err;
-                for (auto& kv : by_solve) {
-                    finish_locked(reqs[kv.second], HMHD_REQ_FAILED);
-                    (void)eng.remove(kv.first);
-                    run.forget(kv.first);
+                // a device error leaves every solve of the lane undefined
+                std::lock_guard<std::mutex> lk(mu);
+                err = L.eng.err;
+                for (auto& kv : L.by_solve) {
+                    Req& r = reqs[kv.second];
+                    finish_locked(r, HMHD_REQ_FAILED);
+                    release_locked(L, r);
+                    (void)L.eng.remove(kv.first);
+                    L.run.forget(kv.first);
                 }
-                by_solve.clear();
+                L.by_solve.clear();
+                admit_locked();
                 cv_done.notify_all();
                 continue;
             }
+            // results of the finished solves, outside the lock: nobody reads
+            // a request's arrays before it is marked done
+            std::vector<std::pair<int64_t, int32_t>> done;
             for (int64_t sid : fin) {
-                Req& r = reqs[by_solve[sid]];
-                const hmhd::RunInfo& ri = run.info[sid];
-                const hmhd::InstHost* h = eng.find(sid);
-                const size_t np = (size_t)h->mx * h->my;
-                r.Uout.resize(8 * np);
-                r.Vout.resize(10 * np);
-                r.Sout.resize(np);
-                int32_t state = ri.state == hmhd::RUN_DONE ? HMHD_REQ_DONE : HMHD_REQ_ABORTED;
-                if (eng.download(sid, r.Uout.data(), r.Vout.data(), r.Sout.data()) != HMHD_OK) {
-                    err = eng.err;
-                    state = HMHD_REQ_FAILED;
+                const int64_t rid = L.by_solve[sid];
+                Req* r;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    r = &reqs[rid];
                 }
-                r.res.steps = ri.steps;
-                r.res.t = ri.t;
-                r.res.zone_updates = ri.zone_updates;
-                finish_locked(r, state);
-                st.completed += 1;
-                st.zone_updates += ri.zone_updates;
-                (void)eng.remove(sid);
-                run.forget(sid);
-                by_solve.erase(sid);
+                const hmhd::RunInfo& ri = L.run.info[sid];
+                const hmhd::InstHost* h = L.eng.find(sid);
+                const size_t np = (size_t)h->mx * h->my;
+                r->Uout.resize(8 * np);
+                r->Vout.resize(10 * np);
+                r->Sout.resize(np);
+                int32_t state = ri.state == hmhd::RUN_DONE ? HMHD_REQ_DONE : HMHD_REQ_ABORTED;
+                if (L.eng.download(sid, r->Uout.data(), r->Vout.data(), r->Sout.data()) != HMHD_OK)
+                    state = HMHD_REQ_FAILED;
+                r->res.steps = ri.steps;
+                r->res.t = ri.t;
+                r->res.zone_updates = ri.zone_updates;
+                done.emplace_back(rid, state);
+                (void)L.eng.remove(sid);
+                L.run.forget(sid);
+                L.by_solve.erase(sid);
             }
-            if (!fin.empty()) cv_done.notify_all();
+            std::lock_guard<std::mutex> lk(mu);
+            busy_s += b - a;
+            st.engine_steps += 1;
+            for (auto& [rid, state] : done) {
+                Req& r = reqs[rid];
+                if (state == HMHD_REQ_FAILED) err = L.eng.err;
+                finish_locked(r, state);
+                release_locked(L, r);
+                st.completed += 1;
+                st.zone_updates += r.res.zone_updates;
+            }
+            if (!done.empty()) {
+                admit_locked();
+                cv_done.notify_all();
+            }
         }
     }
 };
@@ -486,7 +567,14 @@ int hmhd_server_create(uint64_t hbm_budget_bytes, int32_t max_active, hmhd_serve
     hmhd_server_t* s = new hmhd_server_handle;
     s->budget = hbm_budget_bytes;
     s->max_active = max_active;
-    s->worker = std::thread([s] { s->loop(); });
+    for (int k = 0; k <= hmhd_server_handle::kSoloLanes; k++) {
+        s->lanes.push_back(std::make_unique<hmhd_server_handle::Lane>());
+        s->lanes.back()->solo = k > 0;
+    }
+    for (auto& l : s->lanes) {
+        hmhd_server_handle::Lane* lp = l.get();
+        l->th = std::thread([s, lp] { s->lane_loop(*lp); });
+    }
     *out = s;
     return HMHD_OK;
 }
@@ -496,9 +584,10 @@ void hmhd_server_destroy(hmhd_server_t* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
+        for (auto& l : s->lanes) l->cv.notify_all();
     }
-    s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (auto& l : s->lanes)
+        if (l->th.joinable()) l->th.join();
     delete s;
 }
 
@@ -532,8 +621,8 @@ int hmhd_server_submit(hmhd_server_t* s, const hmhd_config* cfg, const double* U
         s->queue.push_back(r.id);
         s->reqs.emplace(r.id, std::move(r));
         s->st.submitted += 1;
+        s->admit_locked();
     }
-    s->cv_work.notify_all();
     return HMHD_OK;
 }
 

@@ -42,7 +42,7 @@ enum : int {
 enum : int { MODE_NEWPASS = 0, MODE_INNER = 1 };
 
 constexpr int kMaxR = 3;
-constexpr int kMaxGpSets = 8;
+constexpr int kMaxGpSets = 64;  // 64 x 4.8 KB of constant memory
 // prediction-vector block of radius rr inside one set: [4 faces][nq][ns]
 constexpr int gp_ns(int rr) { return 1 + 2 * rr * (rr + 1); }
 constexpr int gp_nq(int rr) { return rr + 1; }

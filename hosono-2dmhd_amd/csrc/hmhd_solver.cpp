@@ -30,7 +30,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 }  // namespace
 
-int gp_acquire(const double* pv, const double* qw, std::string& err) {
+int gp_acquire(const double* pv, const double* qw, hipStream_t st, std::string& err) {
     std::lock_guard<std::mutex> lk(g_gp_mu);
     int free_set = -1;
     for (int s = 0; s < kMaxGpSets; s++) {
@@ -42,14 +42,15 @@ int gp_acquire(const double* pv, const double* qw, std::string& err) {
         if (g_gp[s].refs == 0 && free_set < 0) free_set = s;
     }
     if (free_set < 0) {
-        err = "more than 8 distinct GP table sets in one process";
-        return -1;
+        err = "more than 64 distinct GP table sets in one process (every set is in use)";
+        return -2;
     }
     GpSet& g = g_gp[free_set];
     std::memcpy(g.pv, pv, sizeof g.pv);
     std::memcpy(g.qw, qw, sizeof g.qw);
-    set_gp_tables(free_set, g.pv, g.qw, nullptr);
-    if (hipDeviceSynchronize() != hipSuccess) {
+    // on the engine's own stream: other engines' kernels read other sets meanwhile
+    set_gp_tables(free_set, g.pv, g.qw, st);
+    if (hipStreamSynchronize(st) != hipSuccess) {
         err = "uploading GP tables to constant memory failed";
         return -1;
     }
@@ -71,21 +72,37 @@ int Engine::hip_check(hipError_t e, const char* what) {
     return HMHD_ERR_DEVICE;
 }
 
+// Every engine works on a stream of its own that does not synchronize with
+// the null stream, and every copy, allocation and release it makes is
+// ordered on that stream alone, so engines on one device (the server's
+// lanes) overlap.  Solve blocks come from the device's default memory pool
+// (stream-ordered), kept reserved once released so that later solves reuse
+// it without a device-wide synchronization.
 int Engine::ensure_stream() {
     if (st_) return HMHD_OK;
-    return hip_check(hipStreamCreate(&st_), "hipStreamCreate");
+    static std::once_flag pool_once;
+    std::call_once(pool_once, [] {
+        int dev = 0;
+        hipMemPool_t pool;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    });
+    return hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
 }
 
 Engine::~Engine() {
     if (st_) (void)hipStreamSynchronize(st_);
     for (auto& in : inst_) {
-        if (in.block) (void)hipFree(in.block);
+        if (in.block) (void)hipFreeAsync(in.block, st_);
         gp_release(in.gpset);
     }
     void* dev[] = {d_insts_, d_ctl_,   d_dt_,    d_pad_,    d_cell_,
                    d_act_,   d_marks_, d_tbad_, d_part_, d_stage_, d_stage_i_};
     for (void* p : dev)
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, st_);
+    if (st_) (void)hipStreamSynchronize(st_);
     void* host[] = {h_ctl_, h_cfl_, h_dt_, h_scheme_};
     for (void* p : host)
         if (p) (void)hipHostFree(p);
@@ -152,12 +169,13 @@ int64_t Engine::add(const hmhd_config& c) {
         err = "GP covariance matrix is not positive definite (src/linAlgQuadPrecision.f90:136)";
         return -1;
     }
-    h.gpset = gp_acquire(h.pv, h.qw, err);
+    h.gpset = gp_acquire(h.pv, h.qw, st_, err);
+    gp_full = h.gpset == -2;
     if (h.gpset < 0) return -1;
     h.bytes = footprint(c);
-    if (hip_check(hipMalloc(&h.block, h.bytes), "hipMalloc(solve)") ||
+    if (hip_check(hipMallocAsync(&h.block, h.bytes, st_), "hipMallocAsync(solve)") ||
         hip_check(hipMemsetAsync(h.block, 0, h.bytes, st_), "hipMemsetAsync(solve)")) {
-        if (h.block) (void)hipFree(h.block);
+        if (h.block) (void)hipFreeAsync(h.block, st_);
         gp_release(h.gpset);
         return -1;
     }
@@ -235,7 +253,7 @@ int Engine::remove(int64_t id) {
     int rc = sync();
     if (rc) return rc;
     InstHost& h = inst_[it->second];
-    if (h.block) (void)hipFree(h.block);
+    if (h.block) (void)hipFreeAsync(h.block, st_);
     gp_release(h.gpset);
     bytes_used_ -= h.bytes;
     inst_.erase(inst_.begin() + it->second);
@@ -269,11 +287,11 @@ int Engine::grow(T** p, size_t& cap, size_t need, bool host, unsigned host_flags
         if (host)
             (void)hipHostFree(*p);
         else
-            (void)hipFree(*p);
+            (void)hipFreeAsync(*p, st_);
         *p = nullptr;
     }
     int rc = host ? hip_check(hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * n, host_flags), "hipHostMalloc")
-                  : hip_check(hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * n), "hipMalloc");
+                  : hip_check(hipMallocAsync(reinterpret_cast<void**>(p), sizeof(T) * n, st_), "hipMallocAsync");
     if (rc) return rc;
     cap = n;
     return HMHD_OK;
@@ -351,13 +369,14 @@ int Engine::commit() {
     if ((rc = hip_check(hipMemsetAsync(d_tbad_, 0, sizeof(uint32_t) * (ntiles + 1), st_), "hipMemsetAsync(tbad)")))
         return rc;
     for (size_t k = 0; k < n; k++) inst_[k].dev.tbad = table[k].tbad = d_tbad_ + table[k].tile_off;
-    if (n && (rc = hip_check(hipMemcpy(d_insts_, table.data(), sizeof(Inst) * n, hipMemcpyHostToDevice),
-                             "hipMemcpy(table)")))
+    if (n && (rc = hip_check(hipMemcpyAsync(d_insts_, table.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, st_),
+                             "hipMemcpyAsync(table)")))
         return rc;
     auto put = [&](const std::vector<Item>& v, Item** d, size_t& cap) -> int {
         int r = grow(d, cap, v.size() + 1, false);
         if (r || v.empty()) return r;
-        return hip_check(hipMemcpy(*d, v.data(), sizeof(Item) * v.size(), hipMemcpyHostToDevice), "hipMemcpy(items)");
+        return hip_check(hipMemcpyAsync(*d, v.data(), sizeof(Item) * v.size(), hipMemcpyHostToDevice, st_),
+                         "hipMemcpyAsync(items)");
     };
     if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
     if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
@@ -404,8 +423,13 @@ int Engine::set_gp(int64_t id, const double* pv, const double* qw) {
         err = "unknown solve id";
         return HMHD_ERR_ARG;
     }
-    int s = gp_acquire(pv, qw, err);
+    int rc = ensure_stream();
+    if (rc) return rc;
+    int s = gp_acquire(pv, qw, st_, err);
     if (s < 0) return HMHD_ERR_DEVICE;
+    // work already queued (cfl()'s first pass) may still read the old set;
+    // once released, another engine may overwrite it from its own stream
+    if ((rc = sync())) return rc;
     gp_release(h->gpset);
     h->gpset = s;
     h->dev.gpset = s;
@@ -690,13 +714,15 @@ int Engine::upload(int64_t id, const double* U, const double* V) {
     int rc = commit();  // staging buffers sized for every solve
     if (rc) return rc;
     const size_t np = (size_t)h->mx * h->my;
-    if ((rc = hip_check(hipMemcpy(d_stage_, U, sizeof(double) * NCONS * np, hipMemcpyHostToDevice), "hipMemcpy(U)")))
+    if ((rc = hip_check(hipMemcpyAsync(d_stage_, U, sizeof(double) * NCONS * np, hipMemcpyHostToDevice, st_),
+                        "hipMemcpyAsync(U)")))
         return rc;
     launch_unpack(h->dev, d_stage_, NCONS, false, h->dev.rot, st_);
     if ((rc = sync())) return rc;
     h->upU.assign(U, U + NCONS * np);
     if (V) {
-        if ((rc = hip_check(hipMemcpy(d_stage_, V, sizeof(double) * 10 * np, hipMemcpyHostToDevice), "hipMemcpy(V)")))
+        if ((rc = hip_check(hipMemcpyAsync(d_stage_, V, sizeof(double) * 10 * np, hipMemcpyHostToDevice, st_),
+                            "hipMemcpyAsync(V)")))
             return rc;
         launch_unpack(h->dev, d_stage_, 10, true, 0, st_);
         if ((rc = sync())) return rc;

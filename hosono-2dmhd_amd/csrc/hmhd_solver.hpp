@@ -22,8 +22,9 @@
 
 namespace hmhd {
 
-// process-wide registry of GP table sets held in __constant__ memory
-int gp_acquire(const double* pv, const double* qw, std::string& err);
+// process-wide registry of GP table sets held in __constant__ memory;
+// gp_acquire returns the set, -1 on a device error, -2 when every set is in use
+int gp_acquire(const double* pv, const double* qw, hipStream_t st, std::string& err);
 void gp_release(int set);
 
 struct InstHost {
@@ -84,6 +85,7 @@ class Engine {
     size_t size() const { return inst_.size(); }
 
     std::string err;
+    bool gp_full = false;  // the last add() failed only because every GP table set is in use
     bool record_schemes = false;
     uint32_t prof_mask = 0;  // bit f: time kernel family f
     double prof_ms[PF_N] = {0};

@@ -187,10 +187,14 @@ int hmhd_batch_profile_read(hmhd_batch_t* b, double* ms, int64_t* launches, int3
 /* ------------------------------------------------------------------ *
  * Server: runs submitted at any time are admitted in arrival order while
  * their device footprint fits the HBM budget (later, smaller requests are
- * admitted past one that does not fit yet), stepped together with every
- * other admitted run, and released when they finish.  A worker thread owns
- * the device; the calls below are thread-safe.  Results are bit-identical
- * to the same run on its own handle.  The reference has no served form.
+ * admitted past one that does not fit yet), kept resident on the device,
+ * stepped with the other admitted runs, and released when they finish.
+ * Admitted runs go to lanes that overlap on the device, each an engine
+ * with its own stream and worker thread: one lane steps any number of runs
+ * together, two more hold one large run (512^2 cells or more) each.  The
+ * calls below are thread-safe.  Results are bit-identical to the same run
+ * on its own handle.  A run that finds all GP table sets in use waits for
+ * a running one to finish.  The reference has no served form.
  */
 typedef struct hmhd_server_handle hmhd_server_t;
 
@@ -212,9 +216,9 @@ typedef struct hmhd_server_stats {
     int64_t submitted, completed;
     int32_t active, queued, peak_active, reserved;
     uint64_t bytes_in_use, peak_bytes; /* device bytes held by admitted runs */
-    int64_t engine_steps;              /* steps of the shared engine */
+    int64_t engine_steps;              /* engine steps, summed over the lanes */
     int64_t zone_updates;              /* of completed runs */
-    double busy_s, elapsed_s;          /* time stepping; time since start */
+    double busy_s, elapsed_s;          /* time stepping, summed over the lanes; time since start */
 } hmhd_server_stats;
 
 /* Device bytes one run of this configuration holds while admitted. */

@@ -164,3 +164,30 @@ def test_server_with_one_slot_runs_requests_in_order():
     assert res[0].done_s <= res[1].admit_s
     for k, x in zip((3, 4), res):
         _check(k, x.t, x.steps, x.U, x.V, f"request {k}")
+
+
+def test_server_with_many_distinct_gp_table_sets():
+    """Ten runs whose GP tables all differ (l / dx from 6 to 15, four
+    aspect ratios dy / dx) through one server at once: each equals its
+    oracle run bit for bit."""
+    import hosono_2dmhd_amd.ics as ics
+    runs = []
+    for k in range(10):
+        nx, ny, yend = 24, 20 + 2 * (k % 4), 1.0 + 0.1 * (k % 4)
+        g = ics.Grid(nx, ny, 0.0, 1.0, 0.0, yend, R=2)
+        U, V = ics.orszag_tang(g)
+        cfg = O.Config(nx, ny, 0.0, 1.0, 0.0, yend, R=2, ell_over_delta=6.0 + k)
+        runs.append((cfg, U, V))
+    tables = {cases.runtime_config(c).ell_over_delta for c, _, _ in runs}
+    assert len(tables) == 10
+    with rt.Server(1 << 34) as srv:
+        reqs = [srv.submit(cases.runtime_config(c), U, V, 1.0, 5) for c, U, V in runs]
+        res = [srv.wait(r, timeout_s=120.0) for r in reqs]
+    for k, ((c, U, V), x) in enumerate(zip(runs, res)):
+        orc = O.Oracle(c)
+        orc.set_state(U, V)
+        t, steps = orc.run(5, 1.0)
+        Uo, Vo = orc.get_state()
+        assert x.state == rt.REQ_DONE, f"run {k}: state {x.state}"
+        assert (x.t, x.steps) == (t, steps), f"run {k}"
+        assert np.array_equal(x.U, Uo) and np.array_equal(x.V, Vo), f"run {k}: state differs from the oracle"
