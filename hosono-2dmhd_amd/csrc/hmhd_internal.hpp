@@ -25,7 +25,7 @@ namespace hmhd {
 enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1 };
 enum : int { NCONS = 8 };
 // ctl words: flags the kernels raise, the MOOD-loop state k_stage_init and
-// the loop decisions (in k_tile_scan's launch) keep on the device, and
+// the loop decisions (in k_end_iter's launch) keep on the device, and
 // per-stage pass / inner-iteration counters
 enum : int {
     CTL_ANY_INADM = 0,  // an inadmissible reconstructed state was found
@@ -37,6 +37,7 @@ enum : int {
     CTL_SYNC = 10,      // this pass ends with the periodic MOOD sync (set before the loop decisions)
     CTL_LOWER = 11,     // the next iteration is an inner-loop round, lowered in this pass's scan launch
     CTL_FIRST = 12,     // no pass of this stage has run its update yet: rc and rcF are set everywhere
+    CTL_READY = 13,     // this iteration's pass ran its update and detect (set before the loop decisions)
     CTL_N = 16
 };
 enum : int { MODE_NEWPASS = 0, MODE_INNER = 1 };
@@ -116,6 +117,7 @@ struct LaunchSet {
     int32_t* act_count;
     const int32_t* npad_dev;  // npad, on the device (the length of `pad` as a TileList)
     uint32_t* marks;     // per tile: stamp of the last iteration that may change its flags
+    uint32_t* amarks;    // per tile: stamp of the last iteration whose update visited it
     unsigned long long* cflpart;  // two partial maxima per CFL block
     // One solve in the table: the per-iteration kernels take its record
     // from their arguments (`inst1`, as the device table holds it when the

@@ -10,10 +10,12 @@
 //              the RK combination of the updated cell (src/FLICDdivB.f90,
 //              src/timeStep.f90:107-118, src/detect.f90:74-217,
 //              src/timeStep.f90:165,169)
-//   k_lower_*  the order-lowering bookkeeping of src/gridBlock.f90:194-235
-//              written as a gather, so no cell is written by two threads
-//   k_tile_scan  the periodic MOOD sync (src/boundaryConditions.f90:189-254),
-//              the tiles the next iteration needs, and the loop decisions
+//   k_end_iter the order-lowering bookkeeping after detect
+//              (src/gridBlock.f90:194-235, written as a gather, so no cell is
+//              written by two threads), the periodic MOOD sync
+//              (src/boundaryConditions.f90:189-254), the tiles the next
+//              iteration needs, the next inner-loop round's lowering and the
+//              loop decisions
 //
 // The first pass of a stage runs k_face in FULL mode: every face at the top
 // radius, admissibility checked on the way.  Every later iteration (a new
@@ -759,72 +761,94 @@ __device__ __forceinline__ void lower_cell(const Inst& I, int i, int j) {
     I.rec[idx] = cross;
 }
 
-// after detect: rc = cross(low), rcF = diamond(low), scheme lowered
-// (src/detect.f90:60-72,219-234)
-__device__ __forceinline__ void lower_detect_tile(const Inst& I, const Ctl& cw, const Item it) {
-    if (!pass_ready(cw)) return;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
-    if (i >= I.mx || j >= I.my) return;
-    const size_t idx = (size_t)j * I.mx + i;
-    const uint8_t low = I.low[idx], sc = I.scheme[idx];  // loads before stores
-    uint8_t cross, diam;
-    gather_or(I.low, nbhd(i, j, I.mx, I.my), cross, diam);
-    I.rc[idx] = cross;
-    I.rcF[idx] = diam;
-    if (low) I.scheme[idx] = (uint8_t)(sc > 0 ? sc - 1 : 0);
+// Band cells of a periodic solve: the guard columns and rows and their
+// interior images N cells away, [0, 2 NGC) and [N, N + 2 NGC) along each
+// axis (disjoint for N >= 2 NGC).  The periodic MOOD sync owns them.
+__device__ __forceinline__ bool band_cell(const Inst& I, int i, int j) {
+    const int s2 = 2 * I.ngc;
+    return i < s2 || i >= I.nx || j < s2 || j >= I.ny;
 }
 
-template <bool ONE>
-__global__ __launch_bounds__(256) void k_lower_detect(const Inst* __restrict__ insts, const Inst one,
-                                                      const TileList tl, int ninst) {
-    const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
-    // block 0 records, for the scan launch, whether this pass ends with the
-    // periodic sync and whether the next iteration is an inner-loop round
-    // whose lowering that launch does (its loop decisions rewrite the words
-    // these come from)
-    if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
-            const Inst& I = inst_at<ONE>(insts, one, k);
-            const Ctl c = ONE ? c1 : ctl_of(I.ctl);
-            I.ctl[CTL_SYNC] = sync_due(I, c) ? 1 : 0;
-            I.ctl[CTL_LOWER] = c.run != 0 && c.any_inadm != 0 ? 1 : 0;
-            if (pass_ready(c)) I.ctl[CTL_FIRST] = 0;  // this launch replaces rc and rcF
+// The post-detect lowering (src/detect.f90:60-72,219-234) of the cells
+// lane, lane + 64, lane + 128, lane + 192 of one tile, as a gather: rc =
+// cross(low), rcF = diamond(low), the scheme lowered where low.  Every load
+// of the four cells is issued before the first store.  Band cells are left
+// to the sync when `skip_bands` (a periodic solve whose pass syncs).
+__device__ __forceinline__ void lower_detect_wave(const Inst& I, const Item it, int lane, bool skip_bands) {
+    constexpr int NC = kCellT * kCellT / 64;
+    uint8_t own[NC], cross[NC], diam[NC], sc[NC];
+    bool on[NC];
+    size_t idx[NC];
+#pragma unroll
+    for (int r = 0; r < NC; r++) {
+        const int c = lane + 64 * r, i = it.tx * kCellT + c % kCellT, j = it.ty * kCellT + c / kCellT;
+        on[r] = i < I.mx && j < I.my && !(skip_bands && band_cell(I, i, j));
+        const int ci = i < I.mx ? i : I.mx - 1, cj = j < I.my ? j : I.my - 1;
+        idx[r] = (size_t)cj * I.mx + ci;
+        const Nbhd nb = nbhd(ci, cj, I.mx, I.my);
+        uint8_t cr = 0, dm = 0;
+#pragma unroll
+        for (int k = 0; k < 13; k++) {
+            const uint8_t x = nb.in[k] ? I.low[nb.off[k]] : 0;
+            if (k == 0) own[r] = x;
+            if (k < 5) cr |= x;
+            dm |= x;
         }
+        cross[r] = cr;
+        diam[r] = dm;
+        sc[r] = I.scheme[idx[r]];
     }
-    for_tiles(tl, [&](const Item it) {
-        const Inst& I = inst_at<ONE>(insts, one, it.inst);
-        lower_detect_tile(I, ONE ? c1 : ctl_of(I.ctl), it);
-    }, c1.run | c1.any_inadm | c1.mode, ONE && !pass_ready(c1));
+#pragma unroll
+    for (int r = 0; r < NC; r++) {
+        if (!on[r]) continue;
+        I.rc[idx[r]] = cross[r];
+        I.rcF[idx[r]] = diam[r];
+        if (own[r]) I.scheme[idx[r]] = (uint8_t)(sc[r] > 0 ? sc[r] - 1 : 0);
+    }
 }
 
-// Periodic MOOD sync (src/boundaryConditions.f90:205-252) in one launch.
-// The reference sweeps x over every row, then y over every column, pairing
-// each guard cell with its interior image N cells away (OR of rc and rcF,
-// MIN of scheme).  The pairs fall into independent units: a guard column's
-// pair in a row that is neither a guard row nor a guard row's image (x
-// only), a guard row's pair in such a column (y only), and the quad of a
-// guard column and a guard row with their images, whose two x pairs come
-// before its two y pairs as in the sweeps.  Needs N >= 2 NGC, so the guard
-// and image bands are disjoint.  Runs in the scan launch, sync_gx blocks
-// per solve.
+// Periodic MOOD sync (src/boundaryConditions.f90:205-252), in the launch
+// that ends a pass.  The reference sweeps x over every row, then y over
+// every column, pairing each guard cell with its interior image N cells
+// away (OR of rc and rcF, MIN of scheme).  The pairs fall into independent
+// units: a guard column's pair in a row that is neither a guard row nor a
+// guard row's image (x only), a guard row's pair in such a column (y only),
+// and the quad of a guard column and a guard row with their images, whose
+// two x pairs come before its two y pairs as in the sweeps.  Needs N >= 2
+// NGC, so the guard and image bands are disjoint.  sync_gx blocks per solve.
 // A unit of K cells ends with rc and rcF the OR over the unit and scheme
 // its MIN (for a quad, the x pairs then the y pairs give exactly that).
-// Every value is loaded before any is stored.
+// The lowering after detect runs in the same launch and leaves band cells
+// alone: each member's values are formed here as that lowering would leave
+// them (rc = cross(low), rcF = diamond(low), the scheme lowered where low),
+// from `low`, which no kernel of the launch writes.
 template <int K>
-__device__ __forceinline__ void sync_unit(const Inst& I, const size_t (&c)[K]) {
-    uint8_t rc[K], rcf[K], sc[K];
+__device__ __forceinline__ void sync_unit(const Inst& I, const int (&ci)[K], const int (&cj)[K]) {
+    size_t c[K];
+    uint8_t cr[K], dm[K], own[K], sc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        rc[k] = I.rc[c[k]];
-        rcf[k] = I.rcF[c[k]];
+        c[k] = (size_t)cj[k] * I.mx + ci[k];
+        const Nbhd nb = nbhd(ci[k], cj[k], I.mx, I.my);
+        uint8_t a = 0, d = 0;
+#pragma unroll
+        for (int q = 0; q < 13; q++) {
+            const uint8_t x = nb.in[q] ? I.low[nb.off[q]] : 0;
+            if (q == 0) own[k] = x;
+            if (q < 5) a |= x;
+            d |= x;
+        }
+        cr[k] = a;
+        dm[k] = d;
         sc[k] = I.scheme[c[k]];
     }
-    uint8_t orc = rc[0], orf = rcf[0], m = sc[0];
+    uint8_t orc = 0, orf = 0, m = 0xff;
 #pragma unroll
-    for (int k = 1; k < K; k++) {
-        orc |= rc[k];
-        orf |= rcf[k];
-        m = sc[k] < m ? sc[k] : m;
+    for (int k = 0; k < K; k++) {
+        const uint8_t l = own[k] ? (uint8_t)(sc[k] > 0 ? sc[k] - 1 : 0) : sc[k];
+        orc |= cr[k];
+        orf |= dm[k];
+        m = l < m ? l : m;
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -837,7 +861,6 @@ __device__ __forceinline__ void sync_unit(const Inst& I, const size_t (&c)[K]) {
 // block bx of gx over the sync units of one solve
 __device__ __forceinline__ void sync_block(const Inst& I, int bx, int gx) {
     const int s = I.ngc, g2 = 2 * s, nx = I.nx, ny = I.ny;
-    const size_t mx = (size_t)I.mx;
     const int nq = g2 * g2, nxo = g2 * (ny - g2), nyo = g2 * (nx - g2);
     // guard column / row index g in [0, 2s) -> its padded index and image
     auto col = [&](int g, int& i, int& ip) {
@@ -853,20 +876,20 @@ __device__ __forceinline__ void sync_block(const Inst& I, int bx, int gx) {
         if (u < nq) {
             col(u % g2, i, ip);
             row(u / g2, j, jp);
-            const size_t c[4] = {j * mx + i, j * mx + ip, jp * mx + i, jp * mx + ip};
-            sync_unit<4>(I, c);
+            const int xs[4] = {i, ip, i, ip}, ys[4] = {j, j, jp, jp};
+            sync_unit<4>(I, xs, ys);
         } else if (u < nq + nxo) {
             const int v = u - nq;
             col(v % g2, i, ip);
-            const size_t r = (size_t)(g2 + v / g2);
-            const size_t c[2] = {r * mx + i, r * mx + ip};
-            sync_unit<2>(I, c);
+            const int r = g2 + v / g2;
+            const int xs[2] = {i, ip}, ys[2] = {r, r};
+            sync_unit<2>(I, xs, ys);
         } else {
             const int v = u - nq - nxo;
             row(v % g2, j, jp);
-            const size_t cc = (size_t)(g2 + v / g2);
-            const size_t c[2] = {j * mx + cc, jp * mx + cc};
-            sync_unit<2>(I, c);
+            const int cc = g2 + v / g2;
+            const int xs[2] = {cc, cc}, ys[2] = {j, jp};
+            sync_unit<2>(I, xs, ys);
         }
     }
 }
@@ -1085,8 +1108,23 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
 
 template <bool ONE>
 __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
-                                                int stage, uint32_t* __restrict__ marks, uint32_t stamp) {
+                                                int stage, uint32_t* __restrict__ marks,
+                                                uint32_t* __restrict__ amarks, uint32_t stamp, int ninst) {
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
+    // Block 0 records, for the launch that ends the iteration, whether the
+    // pass is ready (its update and detect run here), whether it ends with
+    // the periodic sync, and whether the next iteration is an inner-loop
+    // round whose lowering that launch does: its loop decisions rewrite the
+    // words these come from while its other blocks read them.
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+            const Inst& I = inst_at<ONE>(insts, one, k);
+            const Ctl c = ONE ? c1 : ctl_of(I.ctl);
+            I.ctl[CTL_READY] = pass_ready(c) ? 1 : 0;
+            I.ctl[CTL_SYNC] = sync_due(I, c) ? 1 : 0;
+            I.ctl[CTL_LOWER] = c.run != 0 && c.any_inadm != 0 ? 1 : 0;
+        }
+    }
     // a single solve before its first update, in a pass that is not ready
     // either: the scan that ends the iteration lists every tile (it reads
     // no candidate marks), so none are written
@@ -1108,8 +1146,11 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
         const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
         if (stamp != 0) {
             // a later iteration's active tiles: this is the kernel that
-            // visits all of them, so it marks the scan candidates
+            // visits all of them, so it marks the scan candidates, and, in
+            // a ready pass, the tiles whose cells the lowering after detect
+            // replaces
             if (cw.run && !nomarks) mark_candidates(I, it, marks, stamp);
+            if (threadIdx.x == 0 && pass_ready(cw)) amarks[I.tile_off + it.ty * I.ntx + it.tx] = stamp;
         } else {
             // the first pass, every tile: the stage set-up's scheme = R
             // (src/timeStep.f90:79), read first here
@@ -1156,35 +1197,73 @@ __device__ __forceinline__ void publish_snapshot(int32_t* __restrict__ snap, int
     }
 }
 
-// Active tiles of the next MOOD iteration: every quantity a pass writes
-// (inner lowering, reconstruction, fluxes, update, detect bookkeeping) lies
-// within two cells of a cell flagged rc, rcF or inadm, so a tile takes part
-// when its footprint grown by two cells holds such a cell.  Appended in any
-// order: each tile's work is independent of the others'.
+// The launch that ends a MOOD iteration, k_end_iter:
+//
+// * the lowering after detect (src/detect.f90:60-72,219-234) on the tiles
+//   the iteration's update visited, when the pass was ready;
+// * the periodic MOOD sync, computed from `low` (sync_unit);
+// * the active tiles of the next iteration: every quantity a pass writes
+//   (inner lowering, reconstruction, fluxes, update, detect bookkeeping)
+//   lies within two cells of a cell flagged rc, rcF or inadm, so a tile
+//   takes part when its footprint grown by two cells holds such a cell;
+//   appended in any order, as each tile's work is independent of the
+//   others';
+// * the next inner-loop round's lowering, when the pass was not ready;
+// * the loop decisions and the snapshot for the host.
+//
+// After a ready pass the lowering replaces rc and rcF by cross(low) and
+// diamond(low) on every tile the update visited, and leaves them clear on
+// every other (no cell of those has low within two cells: low is only set
+// where rcF was, and the update's tiles cover every cell within two of an
+// rcF cell).  So every flag after this launch is diamond(low), synced over
+// the periodic units, and the scan reads `low` -- final since the update
+// ended -- instead of flags the lowering waves are writing: a tile is
+// listed when a low cell lies within four cells of it, or of the images of
+// its band cells.
+//
 // kScanTiles tiles per block, one wave each; the block appends its tiles
 // with one atomic, so the list count sees one update per block
 constexpr int kScanTiles = 4;
 //
 // One more block ends the MOOD iteration (the decisions read only flags
-// earlier kernels set, and no scan or sync block reads a word they write):
-// the decisions, the control words copied to `snap`, and, in snap's count
-// slot w and word 3, the lengths of the lists this iteration consumed (list
-// 1 - w, complete since the previous scan, and its face list), whose count
-// it then clears for the next scan; the sequence word `seq` last.  After a
-// first pass, which consumed the fixed list of every tile, the count slots
-// hold nothing the host uses.
-// Five waves per SIMD for the many-solve form (93 VGPRs, no spills): more
-// scan blocks in flight.  The single solve keeps the compiler's own choice
-// (four): at five it spills 14 VGPRs and runs slower.
+// earlier kernels set, and no other block reads a word they write: the
+// words the others read were set by the update's block 0): the decisions,
+// the control words copied to `snap`, and, in snap's count slot w and word
+// 3, the lengths of the lists this iteration consumed (list 1 - w, complete
+// since the previous scan, and its face list), whose count it then clears
+// for the next scan; the sequence word `seq` last.  After a first pass,
+// which consumed the fixed list of every tile, the count slots hold nothing
+// the host uses.
+
+// whether a low cell lies within four cells of the tile at (x0, y0)
+__device__ __forceinline__ bool low_near(const Inst& I, int x0, int y0, int lane) {
+    constexpr int W = kCellT + 8, NR = (W * W + 63) / 64;
+    uint8_t v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int c = lane + r * 64, i = x0 - 4 + c % W, j = y0 - 4 + c / W;
+        const bool in = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
+        const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
+        v[r] = in ? I.low[(size_t)cj * I.mx + ci] : 0;
+    }
+    uint8_t o = 0;
+#pragma unroll
+    for (int r = 0; r < NR; r++) o |= v[r];
+    return o != 0;
+}
+
+// Four waves per SIMD in both forms (about 100-120 VGPRs); at five the
+// many-solve form spills.
 template <bool ONE>
-__global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(const Inst* __restrict__ insts, const Inst one,
-                                                               const Item* __restrict__ items, int npad,
-                                                               Item* __restrict__ act, Item* __restrict__ actf,
-                                                               int32_t* __restrict__ act_count,
-                                                               const uint32_t* __restrict__ marks, uint32_t stamp,
-                                                               int ninst, int stage, int32_t* __restrict__ ctl,
-                                                               int wlist, int32_t* __restrict__ snap, uint32_t seq,
-                                                               int sync_gx) {
+__global__ __launch_bounds__(64 * kScanTiles, 1) void k_end_iter(const Inst* __restrict__ insts, const Inst one,
+                                                              const Item* __restrict__ items, int npad,
+                                                              Item* __restrict__ act, Item* __restrict__ actf,
+                                                              int32_t* __restrict__ act_count,
+                                                              const uint32_t* __restrict__ marks,
+                                                              const uint32_t* __restrict__ amarks, uint32_t stamp,
+                                                              int ninst, int stage, int32_t* __restrict__ ctl,
+                                                              int wlist, int32_t* __restrict__ snap, uint32_t seq,
+                                                              int sync_gx) {
     __shared__ int hit[kScanTiles], hitf[kScanTiles];
     __shared__ int base, basef;
     const int nsync = sync_gx * ninst, nfin = snap != nullptr ? 1 : 0;
@@ -1196,6 +1275,10 @@ __global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(cons
     }
     if ((int)blockIdx.x == nsync && nfin) {
         // the end of the iteration, in a block of its own
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+            int32_t* c = ctl + CTL_N * k;
+            if (c[CTL_READY]) c[CTL_FIRST] = 0;  // this launch replaced rc and rcF
+        }
         loop_decisions(ctl, ninst, stage);
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1216,36 +1299,28 @@ __global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(cons
     const int tc = t < npad ? t : (npad > 0 ? npad - 1 : 0);  // the list has room for one entry at least
     const Item it = ONE ? Item{0, (int16_t)(tc % one.ntx), (int16_t)(tc / one.ntx)} : items[tc];
     const Inst& I = inst_at<ONE>(insts, one, it.inst);
-    bool any = false, anyf = false;
+    const int tile = I.tile_off + it.ty * I.ntx + it.tx;
+    const bool ready = I.ctl[CTL_READY] != 0;
+    const bool sync = I.ctl[CTL_SYNC] != 0;
+    // Before an inner-loop round (CTL_LOWER, never after a ready pass) this
+    // launch lowers for that round: the waves of the face list's tiles raise
+    // rc / rcF / rec within two cells of an inadmissible cell, concurrently
+    // with the other waves' reads.
+    const bool lower = I.ctl[CTL_LOWER] != 0;
     // A single solve before its first update has rc and rcF set everywhere:
     // every tile is active, and takes its own index in the list, without
     // the atomic (with one per block, a full list cost as much as all the
     // rest of the scan)
-    const bool idlist = ONE && I.ctl[CTL_FIRST] != 0;
-    // Before an inner-loop round (CTL_LOWER) this launch also lowers: the
-    // waves of the face list's tiles raise rc / rcF / rec within two cells
-    // of an inadmissible cell, concurrently with the other waves' reads.
-    const bool lower = I.ctl[CTL_LOWER] != 0;
-    if (t < npad && (idlist || stamp == 0 || marks[I.tile_off + it.ty * I.ntx + it.tx] == stamp)) {
-        const int x0 = it.tx * kCellT - 2, y0 = it.ty * kCellT - 2;
-        constexpr int W = kCellT + 4, NR = (W * W + 63) / 64;
-        // rc and rcF of every cell of the footprint, read at once (clamped,
-        // masked), and, in place of the inadmissible sides, the bad_bits of
-        // the 3 x 3 tiles around: every inadmissible side of a footprint
-        // cell was written by one of them
-        uint8_t f[NR];
-        bool in[NR];
+    const bool idlist = ONE && !ready && I.ctl[CTL_FIRST] != 0;
+    if (t < npad && ready && (stamp == 0 || amarks[tile] == stamp)) lower_detect_wave(I, it, lane, sync);
+    bool any = false, anyf = false;
+    if (t < npad && (idlist || stamp == 0 || marks[tile] == stamp)) {
+        // in place of the inadmissible sides, the bad_bits of the 3 x 3
+        // tiles around: every inadmissible side of a footprint cell was
+        // written by one of them
         const int nty = (I.my + kCellT - 1) / kCellT, ta = it.tx + lane % 3 - 1, tb = it.ty + lane / 3 - 1;
         const bool tin = lane < 9 && ta >= 0 && tb >= 0 && ta < I.ntx && tb < nty;
         const uint32_t tf = I.tbad[(tin ? tb : it.ty) * I.ntx + (tin ? ta : it.tx)];
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int c = lane + r * 64, i = x0 + c % W, j = y0 + c / W;
-            in[r] = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
-            const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
-            const size_t idx = (size_t)cj * I.mx + ci;
-            f[r] = I.rc[idx] | I.rcF[idx];
-        }
         // a neighbour's sides count when they lie within two cells of this
         // tile: the edge (or corner) of the neighbour facing it
         const int dx = lane % 3 - 1, dy = lane / 3 - 1;
@@ -1254,31 +1329,44 @@ __global__ __launch_bounds__(64 * kScanTiles, ONE ? 1 : 5) void k_tile_scan(cons
         const uint32_t need4 = BAD_ANY | (dx < 0 ? BAD_E4 : 0u) | (dx > 0 ? BAD_W4 : 0u) | (dy < 0 ? BAD_N4 : 0u) |
                                (dy > 0 ? BAD_S4 : 0u);
         anyf = tin && (tf & need2) == need2;
-        // The lowering raises flags only within two cells of an
-        // inadmissible cell X; the next iteration's update and post-detect
-        // lowering reach two cells further.  Whatever this wave reads of
-        // flags being raised, a tile with a cell within four cells of X is
-        // listed, so every tile the lowered cells make active is.
-        const bool any4 = lower && tin && (tf & need4) == need4;
-        any = anyf || any4 || idlist;
-#pragma unroll
-        for (int r = 0; r < NR; r++) any |= in[r] && f[r] != 0;
-        // The sync of this launch may not have reached the footprint's
-        // guard / image bands yet: a band cell's rc and rcF are read as the
-        // OR over its sync unit, which is their value once synced.
-        const bool bands = I.ctl[CTL_SYNC] && (x0 < 2 * I.ngc || x0 + W > I.nx || y0 < 2 * I.ngc || y0 + W > I.ny);
-        if (bands && !__ballot(any)) {
-            const int s2 = 2 * I.ngc;
-            for (int c = lane; c < W * W; c += 64) {
-                const int i = x0 + c % W, j = y0 + c / W;
-                if (i < 0 || j < 0 || i >= I.mx || j >= I.my) continue;
-                const bool cb = i < s2 || i >= I.nx, rb = j < s2 || j >= I.ny;
-                if (!(cb || rb)) continue;
-                const int ip = cb ? (i < s2 ? i + I.nx : i - I.nx) : i;
-                const int jp = rb ? (j < s2 ? j + I.ny : j - I.ny) : j;
-                const size_t a = (size_t)j * I.mx + ip, b2 = (size_t)jp * I.mx + i, d = (size_t)jp * I.mx + ip;
-                any |= (I.rc[a] | I.rcF[a] | I.rc[b2] | I.rcF[b2] | I.rc[d] | I.rcF[d]) != 0;
+        const int X0 = it.tx * kCellT, Y0 = it.ty * kCellT;
+        if (ready) {
+            any = anyf || low_near(I, X0, Y0, lane);
+            // a band cell of the footprint ends the launch with the OR over
+            // its sync unit: its images' low cells count too
+            if (sync && !__ballot(any)) {
+                const int s2 = 2 * I.ngc;
+                const int sx[3] = {0, X0 - 2 < s2 ? I.nx : 0, X0 + kCellT + 1 >= I.nx ? -I.nx : 0};
+                const int sy[3] = {0, Y0 - 2 < s2 ? I.ny : 0, Y0 + kCellT + 1 >= I.ny ? -I.ny : 0};
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        if ((a == 0 || sx[a] != 0) && (b == 0 || sy[b] != 0) && (a | b) != 0)
+                            any |= low_near(I, X0 + sx[a], Y0 + sy[b], lane);
             }
+        } else {
+            // rc and rcF of every cell of the footprint, read at once
+            // (clamped, masked)
+            const int x0 = X0 - 2, y0 = Y0 - 2;
+            constexpr int W = kCellT + 4, NR = (W * W + 63) / 64;
+            uint8_t f[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int c = lane + r * 64, i = x0 + c % W, j = y0 + c / W;
+                const bool in = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
+                const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
+                const size_t idx = (size_t)cj * I.mx + ci;
+                f[r] = in ? (uint8_t)(I.rc[idx] | I.rcF[idx]) : 0;
+            }
+            // The lowering raises flags only within two cells of an
+            // inadmissible cell X; the next iteration's update and
+            // post-detect lowering reach two cells further.  Whatever this
+            // wave reads of flags being raised, a tile with a cell within
+            // four cells of X is listed, so every tile the lowered cells
+            // make active is.
+            const bool any4 = lower && tin && (tf & need4) == need4;
+            any = anyf || any4 || idlist;
+#pragma unroll
+            for (int r = 0; r < NR; r++) any |= f[r] != 0;
         }
     }
     const bool hw = __ballot(any) != 0;
@@ -1550,13 +1638,13 @@ void launch_stage_init(const LaunchSet& L, int stage, hipStream_t st) {
             hipLaunchKernelGGL(K<false>, grid, block, 0, st, L.insts, L.inst1, __VA_ARGS__); \
     } while (0)
 
-void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
-                      hipStream_t st) {
+void launch_end_iter(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
+                     hipStream_t st) {
     const int nscan = L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1;
     const int sync_gx = L.sync_gx > 0 ? L.sync_gx : 0;
-    HMHD_LAUNCH1(k_tile_scan, g1(sync_gx * L.ninst + (snap ? 1 : 0) + nscan), dim3(64 * kScanTiles), L.pad, L.npad,
-                 L.act[w], L.actf[w], L.act_count + 2 * w, L.marks, stamp, L.ninst, stage, L.ctl, w, snap, seq,
-                 sync_gx);
+    HMHD_LAUNCH1(k_end_iter, g1(sync_gx * L.ninst + (snap ? 1 : 0) + nscan), dim3(64 * kScanTiles), L.pad, L.npad,
+                 L.act[w], L.actf[w], L.act_count + 2 * w, L.marks, L.amarks, stamp, L.ninst, stage, L.ctl, w, snap,
+                 seq, sync_gx);
 }
 
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
@@ -1587,12 +1675,8 @@ void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, in
 }
 
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st) {
-    if (grid <= 0) return;
-    HMHD_LAUNCH1(k_update, g1(grid), dim3(256), tl, stage, L.marks, stamp);
-}
-
-void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st) {
-    HMHD_LAUNCH1(k_lower_detect, g1(grid > 0 ? grid : 1), dim3(256), tl, L.ninst);
+    // block 0 always runs: it writes the control words the iteration's end reads
+    HMHD_LAUNCH1(k_update, g1(grid > 0 ? grid : 1), dim3(256), tl, stage, L.marks, L.amarks, stamp, L.ninst);
 }
 #undef HMHD_LAUNCH1
 

@@ -12,23 +12,26 @@ void set_gp_tables(int set, const double* pv, const double* qw, hipStream_t st);
 // start of stage `stage`: the MOOD loop's control words (the first pass's
 // scan writes list 0) and the flags' stage set-up
 void launch_stage_init(const LaunchSet& L, int stage, hipStream_t st);
+// The launch that ends a MOOD iteration: the lowering after detect (on the
+// update's tiles, when the pass was ready), the periodic MOOD sync (blocks
+// ahead of the scan), the scan of the next iteration's tiles, the next
+// inner-loop round's lowering, the loop decisions.
 // `w`: which of the two active lists the scan of this iteration writes
 // stamp 0 scans every tile; otherwise only tiles marked with `stamp`.
-// The scan also ends the MOOD iteration: loop decisions, and the control
-// words copied to `snap` (a device address of pinned host memory) with, in
-// its count slot w (and word 3), the lengths of the lists the iteration
-// consumed (list 1 - w and its face list), then its sequence word set to
-// `seq`.
-// The periodic MOOD sync runs in the same launch (blocks ahead of the scan).
-void launch_scan_sync(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
-                      hipStream_t st);
+// The control words are copied to `snap` (a device address of pinned host
+// memory) with, in its count slot w (and word 3), the lengths of the lists
+// the iteration consumed (list 1 - w and its face list), then its sequence
+// word set to `seq`.
+void launch_end_iter(const LaunchSet& L, int w, uint32_t stamp, int stage, int32_t* snap, uint32_t seq,
+                     hipStream_t st);
 // tile kernels: `grid` blocks stride over the list
 // full: the first pass (every face)
 void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st);
 // stamp != 0 (a later iteration): also marks the iteration's scan
-// candidates; stamp == 0 (the first pass): also resets the scheme to R
+// candidates and the tiles it visits; stamp == 0 (the first pass): also
+// resets the scheme to R.  Block 0 sets the control words the iteration's
+// end reads.
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st);
-void launch_lower_detect(const LaunchSet& L, TileList tl, int grid, hipStream_t st);
 // CFL maxima of every solve into `out` (pinned host memory, two per solve);
 // the sequence word *seqp is set to `seq` once all are there
 void launch_cfl(const LaunchSet& L, int ninst, unsigned long long* out, uint32_t* done, int32_t* seqp, uint32_t seq,

@@ -381,8 +381,10 @@ int Engine::commit() {
     if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
     if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
     if ((rc = grow(&d_act_, cap_act_, 4 * (pad.size() + 1), false))) return rc;
-    if ((rc = grow(&d_marks_, cap_marks_, ntiles + 1, false))) return rc;
-    if ((rc = hip_check(hipMemsetAsync(d_marks_, 0, sizeof(uint32_t) * (ntiles + 1), st_), "hipMemsetAsync(marks)")))
+    // candidate marks, then the update's visit stamps
+    if ((rc = grow(&d_marks_, cap_marks_, 2 * (ntiles + 1), false))) return rc;
+    if ((rc = hip_check(hipMemsetAsync(d_marks_, 0, sizeof(uint32_t) * 2 * (ntiles + 1), st_),
+                        "hipMemsetAsync(marks)")))
         return rc;
     stamp_ = 0;
     if ((rc = grow(&d_part_, cap_part_, 2 * cell.size() + 2, false))) return rc;
@@ -409,6 +411,7 @@ int Engine::commit() {
                         "hipMemcpyAsync(npad)")))
         return rc;
     L.marks = d_marks_;
+    L.amarks = d_marks_ + ntiles + 1;
     L.cflpart = d_part_;
     L.one = n == 1 ? 1 : 0;
     L.inst1 = n == 1 ? table[0] : Inst{};
@@ -556,9 +559,9 @@ int Engine::enqueue_iteration(int s, int it, int gface, int gupd) {
     launch_update(L, act, gupd, s, stamp_, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
-    launch_lower_detect(L, act, gupd, st_);
-    // the sync and the scan, which also ends the iteration (loop decisions, snapshot)
-    launch_scan_sync(L, w, stamp_, s, d_hctl_ + w * snap_stride(), next_seq(w), st_);
+    // the lowering after detect, the sync and the scan, which also end the
+    // iteration (loop decisions, snapshot)
+    launch_end_iter(L, w, stamp_, s, d_hctl_ + w * snap_stride(), next_seq(w), st_);
     prof_end(PF_FLAGS);
     return hip_check(hipGetLastError(), "kernel launch");
 }
@@ -600,8 +603,7 @@ int Engine::run_stage(int s, bool pre) {
     launch_update(L, all, pr > 0 ? quiet : L.npad, s, 0, st_);
     prof_end(PF_UPDATE);
     prof_begin(PF_FLAGS);
-    launch_lower_detect(L, all, pr > 0 ? quiet : L.npad, st_);
-    launch_scan_sync(L, 0, 0, s, d_hctl_, next_seq(0), st_);
+    launch_end_iter(L, 0, 0, s, d_hctl_, next_seq(0), st_);
     prof_end(PF_FLAGS);
     int rc = hip_check(hipGetLastError(), "kernel launch");
     if (rc) return rc;

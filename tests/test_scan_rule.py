@@ -1,4 +1,6 @@
-"""The active-tile rule of the scan launch before an inner-loop round.
+"""The active-tile rules of the launch that ends a MOOD iteration.
+
+Before an inner-loop round:
 
 In that launch the waves of the face-list tiles lower cells (the inner
 admissibility loop's lowering, src/reconstruct.f90:146-171 with
@@ -11,11 +13,19 @@ flag *after* the lowering, whatever order the waves ran in.
 
 This test takes the worst order (every wave reads the flags as they were
 before the lowering) and checks, by brute force over random grids, that
-the kernel's rule (hmhd_kernels.hip, k_tile_scan: flags read, plus the
+the kernel's rule (hmhd_kernels.hip, k_end_iter: flags read, plus the
 per-tile bad_bits words of the 3 x 3 tiles around, within two and, before
 an inner-loop round, within four cells of the facing edge) lists a
 superset of the exact set.  The rule without the within-four bits misses
 a tile (planted check that the test sees the gap).
+
+After a ready pass the same launch lowers after detect (rc = cross(low),
+rcF = diamond(low)) and syncs the periodic units while its scan waves run,
+so the scan reads `low` instead: a tile is listed when a low cell lies
+within four cells of it, or of the images of its band cells.  Brute force:
+flags = diamond(low), OR-ed over each periodic unit (a guard cell and its
+images N cells away), then the exact set of tiles whose grown footprint
+holds a flag must be covered by the rule.
 """
 import numpy as np
 
@@ -155,3 +165,80 @@ def test_rule_without_the_far_bits_misses_a_tile():
     new = listed(rc | rcf, words, lower=True)
     assert not old[1, 1], "the rule without the far bits should miss it"
     assert new[1, 1] and not (need & ~new).any()
+
+
+def synced_flags(low, s, nx, ny, periodic):
+    """diamond(low) after the periodic MOOD sync: every band cell takes the
+    OR over its unit (itself and its images N cells away)."""
+    f = dilate(low, low.shape)
+    if not periodic:
+        return f
+    my, mx = f.shape
+    out = f.copy()
+    for j in range(my):
+        for i in range(mx):
+            bc, br = i < 2 * s or i >= nx, j < 2 * s or j >= ny
+            if not (bc or br):
+                continue
+            pi = (i + nx if i < 2 * s else i - nx) if bc else i
+            pj = (j + ny if j < 2 * s else j - ny) if br else j
+            out[j, i] = f[j, i] | f[j, pi] | f[pj, i] | f[pj, pi]
+    return out
+
+
+def ready_rule(low, s, nx, ny, periodic):
+    """k_end_iter's rule after a ready pass (low_near and its band shifts)."""
+    my, mx = low.shape
+    ntx, nty = -(-mx // T), -(-my // T)
+    out = np.zeros((nty, ntx), dtype=bool)
+
+    def near(X0, Y0):
+        xa, ya = max(0, X0 - 4), max(0, Y0 - 4)
+        return bool(low[ya:max(0, Y0 + T + 4), xa:max(0, X0 + T + 4)].any())
+
+    for ty in range(nty):
+        for tx in range(ntx):
+            X0, Y0 = tx * T, ty * T
+            hit = near(X0, Y0)
+            if periodic and not hit:
+                sx = [0] + ([nx] if X0 - 2 < 2 * s else []) + ([-nx] if X0 + T + 1 >= nx else [])
+                sy = [0] + ([ny] if Y0 - 2 < 2 * s else []) + ([-ny] if Y0 + T + 1 >= ny else [])
+                hit = any(near(X0 + a, Y0 + b) for a in sx for b in sy if (a, b) != (0, 0))
+            out[ty, tx] = hit
+    return out
+
+
+def test_ready_rule_covers_the_synced_flags():
+    rng = np.random.default_rng(777)
+    for trial in range(120):
+        R = int(rng.integers(1, 4))
+        s = R + 2
+        nx, ny = int(rng.integers(2 * s, 60)), int(rng.integers(2 * s, 60))
+        mx, my = nx + 2 * s, ny + 2 * s
+        periodic = bool(rng.integers(2))
+        low = np.zeros((my, mx), dtype=bool)
+        inner = rng.random((ny, nx)) < rng.choice([0.002, 0.01, 0.05])
+        low[s:s + ny, s:s + nx] = inner
+        need = exact(synced_flags(low, s, nx, ny, periodic), -(-mx // T), -(-my // T))
+        got = ready_rule(low, s, nx, ny, periodic)
+        missing = need & ~got
+        assert not missing.any(), f"trial {trial} (R={R}, {nx}x{ny}, periodic={periodic}): {np.argwhere(missing).tolist()}"
+
+
+def test_ready_rule_needs_the_band_images():
+    """A low cell next to the east edge of a periodic grid flags, through
+    the sync, guard cells on the west side: the tile there is listed only
+    through the band shift."""
+    s, nx, ny = 4, 40, 40
+    mx, my = nx + 2 * s, ny + 2 * s
+    low = np.zeros((my, mx), dtype=bool)
+    low[20, nx + s - 1] = True  # last interior column; its image is guard column s - 1
+    need = exact(synced_flags(low, s, nx, ny, True), -(-mx // T), -(-my // T))
+    got = ready_rule(low, s, nx, ny, True)
+    assert need[1, 0] and got[1, 0]
+    no_shift = np.zeros_like(got)
+    for ty in range(got.shape[0]):
+        for tx in range(got.shape[1]):
+            X0, Y0 = tx * T, ty * T
+            no_shift[ty, tx] = bool(low[max(0, Y0 - 4):Y0 + T + 4, max(0, X0 - 4):X0 + T + 4].any())
+    assert not no_shift[1, 0], "without the band shifts the west tile would be missed"
