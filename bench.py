@@ -251,6 +251,16 @@ def batched_form(rt, args) -> dict:
     import numpy as np
     import hosono_2dmhd_amd.workloads as wl
     runs = wl.batched_runs(args.seed, args.batch_runs)
+    # The face kernel's form is tuned once per process and launch size
+    # class, on the first launch that meets it: warm every class this form
+    # and its per-instance comparison launch (one untimed step each).
+    with rt.Batch() as b:
+        for r in runs:
+            b.add(r.cfg, r.U, r.V, r.tmax, 1)
+        b.run(1)
+    for n in sorted({r.n for r in runs}):
+        r = next(x for x in runs if x.n == n)
+        solo_run(rt, r.cfg, r.U, r.V, r.tmax, 1)
     with rt.Batch() as b:
         ids = [b.add(r.cfg, r.U, r.V, r.tmax, args.batch_steps) for r in runs]
         t0 = time.perf_counter()
@@ -576,6 +586,9 @@ def main() -> int:
     if "blast" in forms:
         line["blast"] = blast_form(rt, args)
         print(f"blast: {line['blast']['value']:.4g} zone-updates/s", file=sys.stderr, flush=True)
+    # the first-pass face kernel's forms, as tuned in this process: per GP
+    # radius and launch size class, the form kept and every form's time
+    line["face_tuning"] = rt.face_tuning_table()
     print(json.dumps(line))
     return 0
 

@@ -14,6 +14,7 @@
 
 #include "../../include/hmhd.h"
 #include "gp_tables.hpp"
+#include "hmhd_kernels.hpp"
 #include "hmhd_runner.hpp"
 #include "hmhd_solver.hpp"
 
@@ -468,6 +469,33 @@ int hmhd_synchronize(hmhd_t* h) {
     const int rc = h->eng.sync();
     if (rc) h->err = h->eng.err;
     return rc;
+}
+
+int hmhd_face_tuning(int32_t mode) {
+    if (mode < 0 || mode > 1 + HMHD_FACE_FORMS) return fail(g_create_err, HMHD_ERR_ARG, "hmhd_face_tuning: bad mode");
+    hmhd::face_tuning_set_mode(mode);
+    return HMHD_OK;
+}
+
+int hmhd_face_tuning_read(hmhd_face_tune* out, int32_t max, int32_t* n) {
+    const std::vector<hmhd::FaceTuneEntry> t = hmhd::face_tuning_table();
+    if (n) *n = (int32_t)t.size();
+    for (int32_t k = 0; out && k < max && k < (int32_t)t.size(); k++) {
+        hmhd_face_tune& o = out[k];
+        o.R = t[k].R;
+        o.size_class = t[k].size_class;
+        o.form = t[k].variant;
+        o.tile_x = hmhd::kFaceVariants[t[k].variant].tx;
+        o.tile_y = hmhd::kFaceVariants[t[k].variant].ty;
+        o.waves = t[k].R >= 3 ? 1 : hmhd::kFaceVariants[t[k].variant].waves;
+        for (int v = 0; v < 8; v++) o.ms[v] = t[k].ms[v];
+    }
+    return HMHD_OK;
+}
+
+int hmhd_face_tuning_clear(void) {
+    hmhd::face_tuning_clear();
+    return HMHD_OK;
 }
 
 // ------------------------------------------------------------------ //

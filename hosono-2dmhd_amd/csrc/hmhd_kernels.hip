@@ -225,7 +225,7 @@ __device__ __forceinline__ double* ubuf(const Inst& I, int k) { return k == 0 ? 
 
 // ------------------------------------------------------------------ //
 // stage set-up: rc = rcF = true everywhere (src/timeStep.f90:80-81);
-// inadm cleared; every face at radius R.  The scheme's reset to R
+// inadm cleared; every face at radius R; every tile's bad_bits word cleared.  The scheme's reset to R
 // (src/timeStep.f90:79) is left to the first-pass k_update, the first
 // kernel to read it: the scheme keeps the last step's values until dt is
 // known, though cfl() enqueues the start of the next step's first pass.
@@ -254,6 +254,8 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
     if ((int)blockIdx.x >= npad) return;
     const Item it = items[xcd_tile(blockIdx.x, gridDim.x)];
     const Inst& I = insts[it.inst];
+    // the first pass's face kernel ORs into the tile's bad_bits word
+    if (threadIdx.x == 0) I.tbad[it.ty * I.ntx + it.tx] = 0;
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i >= I.mx || j >= I.my) return;
     const size_t idx = (size_t)j * I.mx + i, plane = (size_t)I.mx * I.my;
@@ -269,26 +271,21 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
 
 
 // ------------------------------------------------------------------ //
-// face kernel, one block per 16 x 16 tile (blocks stride over the tile
-// list).  Padded cell c owns the x-face at i-1/2 (the east state of its
-// west neighbour and its own west state) and the y-face at j-1/2 (north
-// state of the south neighbour, own south state); cells of [s-2, e+3]^2
-// take part (tiles of the padded grid, the same tiles every per-cell kernel
-// uses).
-// FULL, the first pass of a stage: every state of [s-2, e+2]^2 at radius
-// R, checked, and every flux of [s-1, e+2]^2; 256 threads, one cell's two
-// faces each (two blocks per CU overlap one's LDS fill with the other's
-// arithmetic).
-// Later passes (face_tile_iter): one iteration of the inner admissibility
-// loop fused with the flux recomputation, over a compacted list of the
-// faces whose inputs changed; at R <= 2 the block has 512 threads so a
-// sparse pass costs one round of lanes.
+// face kernels.  Padded cell c owns the x-face at i-1/2 (the east state of
+// its west neighbour and its own west state) and the y-face at j-1/2
+// (north state of the south neighbour, own south state); cells of
+// [s-2, e+3]^2 take part.
+// The first pass of a stage (k_face_full): every face at the top radius,
+// admissibility checked on the way, one cell's two faces per thread, in
+// tiles of a tuned shape (see launch_face_full).
+// Later passes (k_face, face_tile_iter): one iteration of the inner
+// admissibility loop fused with the flux recomputation, over a compacted
+// list of the faces whose inputs changed, on the 16 x 16 tiles every
+// per-cell kernel uses (blocks stride over the tile list); at R <= 2 the
+// block has 512 threads so a sparse pass costs one round of lanes.
 // Inadmissibility is recorded per face side in four byte planes, so each
 // byte has a single writer.  Register budget: two waves per SIMD at R <= 2;
 // R = 3 keeps one (its stencils would spill at two).
-#ifndef HMHD_FACE_WAVES
-#define HMHD_FACE_WAVES 2
-#endif
 __host__ __device__ constexpr int face_threads(int R, bool full) { return (R >= 3 || full ? 1 : 2) * kFaceTX * kFaceTY; }
 template <int R>
 constexpr int face_lds_cells() {
@@ -325,16 +322,46 @@ __device__ __forceinline__ void report_bad(const Inst& I, uint32_t* tbad, uint32
     }
 }
 
-template <int R>
-__device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int stage, double* __restrict__ sU,
+// The bad_bits of a wave whose lanes' cells lie in at most two 16 x 16
+// tiles (a first-pass face tile is 16 or 32 cells wide, its rows within one
+// band of 16): each tile's word gets one atomic OR, and ANY_INADM is raised.
+// rx, ry of a lane's bits are relative to its own cell's 16 x 16 tile.
+__device__ __forceinline__ void report_bad_cells(const Inst& I, uint32_t bits, int i, int j) {
+    const unsigned long long has = __ballot(bits != 0);
+    if (!has) return;
+    const int tt = (j / kCellT) * I.ntx + i / kCellT;
+    const int la = __ffsll(has) - 1;
+    const int ta = __shfl(tt, la);
+    const bool inA = bits != 0 && tt == ta, inB = bits != 0 && tt != ta;
+    uint32_t wa = 0, wb = 0;
+#pragma unroll
+    for (int k = 0; k < BAD_NBITS; k++) {
+        wa |= __ballot(inA && ((bits >> k) & 1u)) != 0 ? 1u << k : 0u;
+        wb |= __ballot(inB && ((bits >> k) & 1u)) != 0 ? 1u << k : 0u;
+    }
+    const unsigned long long hb = __ballot(inB);
+    const int lane = (int)(threadIdx.x & 63u);
+    if (lane == la) {
+        I.ctl[CTL_ANY_INADM] = 1;
+        atomicOr(I.tbad + ta, wa);
+    }
+    if (hb && lane == __ffsll(hb) - 1) atomicOr(I.tbad + tt, wb);
+}
+
+// One first-pass face tile of TX x TY cells at (i0, j0), one cell per
+// thread (its x face at i-1/2 and its y face at j-1/2): every state of
+// [s-2, e+2]^2 at radius R, checked, and every flux of [s-1, e+2]^2 (the
+// range src/reconstruct.f90:81-83 and src/getFluxes.f90:26-30 cover).  The
+// per-tile bad_bits words were cleared by the stage set-up.
+template <int R, int TX, int TY>
+__device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, int stage, double* __restrict__ sU,
                                                double* __restrict__ sAcc) {
-    constexpr int TX = kFaceTX, TY = kFaceTY, H = R + 1, NT = face_threads(R, true);
-    constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
+    constexpr int H = R + 1, NT = TX * TY;
+    constexpr int LX = TX + 2 * R + 1, LP = LX * (TY + 2 * R + 1);
 
     if (I.R != R) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-    const int i0 = it.tx * TX, j0 = it.ty * TY;
     const int i = i0 + lx, j = j0 + ly;
     const bool in_dom = i >= s - 2 && i <= ex + 3 && j >= s - 2 && j <= ey + 3;
     const bool flux_cell = in_dom && i >= s - 1 && i <= ex + 2 && j >= s - 1 && j <= ey + 2;
@@ -347,8 +374,6 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
     const bool recC = cellC, recW = cellW, recS = cellS, fx = flux_cell, fy = flux_cell;
     const bool do_x = recC || recW || fx, do_y = recC || recS || fy;
 
-    uint32_t* const tbad = I.tbad + it.ty * I.ntx + it.tx;
-    if (threadIdx.x == 0) *tbad = 0;  // before the barrier, so the later ORs land on it
     // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
     {
         const double* __restrict__ U = ubuf(I, stage_bufs(I.rot, stage).in);
@@ -363,16 +388,15 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
         }
     }
     __syncthreads();
-    if (!(do_x || do_y)) return;
-
-    const Phys ph = phys_of(I);
-    const int set = I.gpset;
-    const int cc = (ly + H) * LX + (lx + H);  // LDS offset of cell (i, j)
-    const LdsAcc<LX, LP> aC{sU + cc}, aW{sU + cc - 1}, aS{sU + cc - LX};
-    uint8_t* const bad = I.inadm;  // planes [face][my][mx]
     uint32_t bits = 0;
-
-    {
+    // the cell's position in its 16 x 16 tile, for bad_bits
+    const int rx = i % kCellT, ry = j % kCellT;
+    if (do_x || do_y) {
+        const Phys ph = phys_of(I);
+        const int set = I.gpset;
+        const int cc = (ly + H) * LX + (lx + H);  // LDS offset of cell (i, j)
+        const LdsAcc<LX, LP> aC{sU + cc}, aW{sU + cc - 1}, aS{sU + cc - LX};
+        uint8_t* const bad = I.inadm;  // planes [face][my][mx]
         constexpr int NQ = gp_nq(R);
         // The flux sums over the Gauss points are kept in LDS (sAcc, a
         // column per thread, stride NT), not in registers: the kernel is at
@@ -399,8 +423,8 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             }
             if (recW) bad[FACE_E * fplane + idx - 1] = badL;
             if (recC) bad[FACE_W * fplane + idx] = badR;
-            if (recW && badL) bits |= bad_bits(lx - 1, ly);
-            if (recC && badR) bits |= bad_bits(lx, ly);
+            if (recW && badL) bits |= bad_bits(rx - 1, ry);
+            if (recC && badR) bits |= bad_bits(rx, ry);
             if (fx) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v * NT];
@@ -430,8 +454,8 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             }
             if (recS) bad[FACE_N * fplane + idx - mx] = badL;
             if (recC) bad[FACE_S * fplane + idx] = badR;
-            if (recS && badL) bits |= bad_bits(lx, ly - 1);
-            if (recC && badR) bits |= bad_bits(lx, ly);
+            if (recS && badL) bits |= bad_bits(rx, ry - 1);
+            if (recC && badR) bits |= bad_bits(rx, ry);
             if (fy) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v * NT];
@@ -440,7 +464,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, const Item it, int
             }
         }
     }
-    report_bad(I, tbad, bits);
+    report_bad_cells(I, bits, i, j);
 }
 
 // One face of a later MOOD iteration on the four lanes of a quad, lane q
@@ -671,24 +695,43 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     report_bad(I, tbad, bits);
 }
 
-template <int R, bool FULL, bool ONE>
-__global__ __launch_bounds__(face_threads(R, FULL), R >= 3 ? 1 : HMHD_FACE_WAVES) void k_face(
-    const Inst* __restrict__ insts, const Inst one, const TileList tl, int stage) {
+template <int R, bool ONE>
+__global__ __launch_bounds__(face_threads(R, false), R >= 3 ? 1 : 2) void k_face(const Inst* __restrict__ insts,
+                                                                                 const Inst one, const TileList tl,
+                                                                                 int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
-    __shared__ double sAcc[FULL ? NCONS * face_threads(R, true) : 1];
-    __shared__ uint32_t lst[FULL ? 1 : 2 * kFaceTX * kFaceTY];
+    __shared__ uint32_t lst[2 * kFaceTX * kFaceTY];
     __shared__ int lcnt[2];
-    const Ctl c1 = ONE && !FULL ? ctl_of(one.ctl) : Ctl{};
+    const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
     bool later = false;
     for_tiles(tl, [&](const Item it) {
         if (later) __syncthreads();  // the previous tile's LDS reads are done
         later = true;
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
-        if (FULL)
-            face_tile_full<R>(I, it, stage, sU, sAcc);
-        else
-            face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
+        face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
     }, c1.run | c1.mode);
+}
+
+// The first pass's face kernel in one of its tuned forms: TX x TY cells a
+// tile, WAVES waves per SIMD the register allocation allows.  Block b takes
+// face tile xcd_tile(b, nitems): for a single solve its tiles in row-major
+// order, computed from the index; otherwise entry `items[...]` of the list
+// of every active solve's face tiles of this shape.
+template <int R, bool ONE, int TX, int TY, int WAVES>
+__global__ __launch_bounds__(TX * TY, WAVES) void k_face_full(const Inst* __restrict__ insts, const Inst one,
+                                                             const Item* __restrict__ items, int stage) {
+    __shared__ double sU[NCONS * (TX + 2 * R + 1) * (TY + 2 * R + 1)];
+    __shared__ double sAcc[NCONS * TX * TY];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    Item it;
+    if (ONE) {
+        const int ntf = (one.mx + TX - 1) / TX;
+        it = Item{0, (int16_t)(t % ntf), (int16_t)(t / ntf)};
+    } else {
+        it = items[t];
+    }
+    const Inst& I = inst_at<ONE>(insts, one, it.inst);
+    face_tile_full<R, TX, TY>(I, it.tx * TX, it.ty * TY, stage, sU, sAcc);
 }
 
 // The 13 cells at taxicab distance <= 2 of (i, j), the cross (distance
@@ -1647,16 +1690,11 @@ void launch_end_iter(const LaunchSet& L, int w, uint32_t stamp, int stage, int32
                  seq, sync_gx);
 }
 
-void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, int stage, hipStream_t st) {
+void launch_face_iter(const LaunchSet& L, TileList tl, int grid, int R, int stage, hipStream_t st) {
     if (grid <= 0) return;
     const dim3 g = g1(grid);
-#define HMHD_FACE(RR, ONE)                                                                                   \
-    if (full)                                                                                                \
-        hipLaunchKernelGGL((k_face<RR, true, ONE>), g, dim3(face_threads(RR, true)), 0, st, L.insts, L.inst1, \
-                           tl, stage);                                                                       \
-    else                                                                                                     \
-        hipLaunchKernelGGL((k_face<RR, false, ONE>), g, dim3(face_threads(RR, false)), 0, st, L.insts,        \
-                           L.inst1, tl, stage);
+#define HMHD_FACE(RR, ONE)                                                                                     \
+    hipLaunchKernelGGL((k_face<RR, ONE>), g, dim3(face_threads(RR, false)), 0, st, L.insts, L.inst1, tl, stage);
 #define HMHD_FACE_R(ONE)   \
     if (R == 1) {          \
         HMHD_FACE(1, ONE)  \
@@ -1672,6 +1710,40 @@ void launch_face(const LaunchSet& L, TileList tl, int grid, int R, bool full, in
     }
 #undef HMHD_FACE_R
 #undef HMHD_FACE
+}
+
+template <int R, bool ONE>
+static void launch_ff(const LaunchSet& L, int v, const Item* items, int n, int stage, hipStream_t st) {
+    const dim3 g = g1(n);
+#define HMHD_FF(TX, TY, W) \
+    hipLaunchKernelGGL((k_face_full<R, ONE, TX, TY, W>), g, dim3(TX * TY), 0, st, L.insts, L.inst1, items, stage)
+    if constexpr (R < 3) {
+        switch (v) {
+            case 0: HMHD_FF(16, 16, 2); return;
+            case 1: HMHD_FF(32, 8, 2); return;
+            case 2: HMHD_FF(32, 16, 2); return;
+            case 3: HMHD_FF(16, 16, 1); return;
+            default: HMHD_FF(32, 8, 1); return;
+        }
+    } else {
+        if (v == 1 || v == 4) HMHD_FF(32, 8, 1);
+        else HMHD_FF(16, 16, 1);
+    }
+#undef HMHD_FF
+}
+
+void launch_face_full(const LaunchSet& L, int R, int v, const Item* items, int n, int stage, hipStream_t st) {
+    if (n <= 0) return;
+    static_assert(kNumFaceVariants == 5, "launch_ff's cases");
+    if (L.one) {
+        if (R == 1) launch_ff<1, true>(L, v, items, n, stage, st);
+        else if (R == 2) launch_ff<2, true>(L, v, items, n, stage, st);
+        else launch_ff<3, true>(L, v, items, n, stage, st);
+    } else {
+        if (R == 1) launch_ff<1, false>(L, v, items, n, stage, st);
+        else if (R == 2) launch_ff<2, false>(L, v, items, n, stage, st);
+        else launch_ff<3, false>(L, v, items, n, stage, st);
+    }
 }
 
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st) {

@@ -65,6 +65,28 @@ void gp_release(int set) {
 }
 
 // ------------------------------------------------------------------ //
+// the first-pass face kernel's tuning table (process-wide)
+namespace {
+std::mutex g_tune_mu;
+std::atomic<int> g_tune_mode{1};
+std::vector<FaceTuneEntry> g_tune;
+}  // namespace
+
+void face_tuning_set_mode(int mode) { g_tune_mode = mode; }
+
+std::vector<FaceTuneEntry> face_tuning_table() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tune;
+}
+
+void face_tuning_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
+int face_size_class(long long cells) { return cells < (1 << 16) ? 0 : cells < (1 << 19) ? 1 : cells < (1 << 21) ? 2 : 3; }
+
+// ------------------------------------------------------------------ //
 
 int Engine::hip_check(hipError_t e, const char* what) {
     if (e == hipSuccess) return HMHD_OK;
@@ -98,8 +120,8 @@ Engine::~Engine() {
         if (in.block) (void)hipFreeAsync(in.block, st_);
         gp_release(in.gpset);
     }
-    void* dev[] = {d_insts_, d_ctl_,   d_dt_,    d_pad_,    d_cell_,
-                   d_act_,   d_marks_, d_tbad_, d_part_, d_stage_, d_stage_i_};
+    void* dev[] = {d_insts_, d_ctl_,   d_dt_,    d_pad_,    d_cell_,     d_act_,      d_marks_,
+                   d_tbad_,  d_part_,  d_stage_, d_stage_i_, d_fitems_[0], d_fitems_[1], d_fitems_[2]};
     for (void* p : dev)
         if (p) (void)hipFreeAsync(p, st_);
     if (st_) (void)hipStreamSynchronize(st_);
@@ -379,6 +401,24 @@ int Engine::commit() {
                          "hipMemcpyAsync(items)");
     };
     if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
+    // first-pass face tiles of each shape; padded cells per radius
+    for (int R = 0; R <= kMaxR; R++) rcells_[R] = 0;
+    for (int k = 0; k < kNumFaceShapes; k++) {
+        const int v = k == 0 ? 0 : (k == 1 ? 1 : 2);
+        const int tx = kFaceVariants[v].tx, ty = kFaceVariants[v].ty;
+        std::vector<Item> f;
+        for (size_t q = 0; q < n; q++) {
+            const InstHost& h = inst_[q];
+            if (!h.active) continue;
+            const int ntx = (h.mx + tx - 1) / tx, nty = (h.my + ty - 1) / ty;
+            for (int b = 0; b < nty; b++)
+                for (int a = 0; a < ntx; a++) f.push_back(Item{(int)q, (int16_t)a, (int16_t)b});
+        }
+        nfitems_[k] = (int)f.size();
+        if ((rc = put(f, &d_fitems_[k], cap_fitems_[k]))) return rc;
+    }
+    for (size_t q = 0; q < n; q++)
+        if (inst_[q].active) rcells_[inst_[q].cfg.R] += (long long)inst_[q].mx * inst_[q].my;
     if ((rc = put(cell, &d_cell_, cap_cell_))) return rc;
     if ((rc = grow(&d_act_, cap_act_, 4 * (pad.size() + 1), false))) return rc;
     // candidate marks, then the update's visit stamps
@@ -536,9 +576,58 @@ int Engine::read_snapshot(int w, bool visited) {
 }
 
 // templated kernels run once per GP radius present (tiles of other radii return)
-void Engine::launch_face_groups(TileList tl, int grid, bool full, int stage) {
+void Engine::launch_face_iter_groups(TileList tl, int grid, int stage) {
     for (int R = 1; R <= kMaxR; R++)
-        if (has_R_[R]) launch_face(L, tl, grid, R, full, stage, st_);
+        if (has_R_[R]) launch_face_iter(L, tl, grid, R, stage, st_);
+}
+
+// The first-pass face kernel's form for the solves of radius R, from the
+// tuning table; on the first launch of a (radius, size class) pair every
+// form runs three times on this first pass (they all compute the same
+// bits: each face's arithmetic does not depend on which thread does it),
+// and the fastest is kept.
+int Engine::face_variant(int R, int stage) {
+    const int dflt = R < 3 ? 0 : 3, mode = g_tune_mode;
+    if (mode == 0) return dflt;
+    if (mode >= 2) return face_variant_valid(R, mode - 2) ? mode - 2 : dflt;
+    const int cls = face_size_class(rcells_[R]);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto& e : g_tune)
+            if (e.R == R && e.size_class == cls) return e.variant;
+    }
+    FaceTuneEntry e;
+    e.R = R;
+    e.size_class = cls;
+    e.variant = dflt;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return dflt;
+    double best = 1e30;
+    for (int v = 0; v < kNumFaceVariants; v++) {
+        if (!face_variant_valid(R, v)) continue;
+        const int k = face_shape_of(v);
+        double t = 1e30;
+        for (int rep = 0; rep < 3; rep++) {
+            (void)hipEventRecord(a, st_);
+            launch_face_full(L, R, v, d_fitems_[k], nfitems_[k], stage, st_);
+            (void)hipEventRecord(b, st_);
+            float ms = 0.f;
+            if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 1e30f;
+            t = ms < t ? ms : t;
+        }
+        e.ms[v] = t;
+        if (t < best) {
+            best = t;
+            e.variant = v;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto& x : g_tune)
+        if (x.R == R && x.size_class == cls) return x.variant;  // another engine measured it meanwhile
+    g_tune.push_back(e);
+    return e.variant;
 }
 
 // MOOD iteration it >= 1 of stage s: the active tiles listed by iteration
@@ -553,7 +642,7 @@ int Engine::enqueue_iteration(int s, int it, int gface, int gupd) {
     if (++stamp_ == 0) stamp_ = 1;
     // (an inner-loop round's lowering ran in the previous scan launch)
     prof_begin(PF_FACE_FLUX);
-    launch_face_groups(actf, gface, false, s);
+    launch_face_iter_groups(actf, gface, s);
     prof_end(PF_FACE_FLUX);
     prof_begin(PF_UPDATE);
     launch_update(L, act, gupd, s, stamp_, st_);
@@ -570,10 +659,15 @@ int Engine::enqueue_iteration(int s, int it, int gface, int gupd) {
 // and the first pass's face kernel (every face at the top radius).
 // Nothing here reads dt or changes the scheme.
 void Engine::first_pass_faces(int s) {
-    const TileList all{L.pad, L.npad_dev, 1};
     launch_stage_init(L, s, st_);
+    int var[kMaxR + 1];
+    for (int R = 1; R <= kMaxR; R++) var[R] = has_R_[R] ? face_variant(R, s) : 0;
     prof_begin(PF_FACE_FULL);
-    launch_face_groups(all, L.npad, true, s);
+    for (int R = 1; R <= kMaxR; R++) {
+        if (!has_R_[R]) continue;
+        const int k = face_shape_of(var[R]);
+        launch_face_full(L, R, var[R], d_fitems_[k], nfitems_[k], s, st_);
+    }
     prof_end(PF_FACE_FULL);
 }
 

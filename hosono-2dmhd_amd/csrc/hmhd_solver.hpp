@@ -22,6 +22,21 @@
 
 namespace hmhd {
 
+// The first pass's face kernel comes in tuned forms (kFaceVariants).  The
+// form for a launch is chosen per (GP radius, size class of the launch):
+// measured on the first first pass that meets the pair, then kept for the
+// process.  mode: 1 measure (default), 0 always the 16 x 16 two-wave form,
+// 2 + v always form v.
+struct FaceTuneEntry {
+    int R = 0, size_class = 0, variant = 0;
+    double ms[8] = {0};  // milliseconds per form, min over three launches (0: not run)
+};
+void face_tuning_set_mode(int mode);
+std::vector<FaceTuneEntry> face_tuning_table();
+void face_tuning_clear();
+// size class of a launch over `cells` padded cells: below 2^16, 2^19, 2^21, or above
+int face_size_class(long long cells);
+
 // process-wide registry of GP table sets held in __constant__ memory;
 // gp_acquire returns the set, -1 on a device error, -2 when every set is in use
 int gp_acquire(const double* pv, const double* qw, hipStream_t st, std::string& err);
@@ -102,7 +117,8 @@ class Engine {
     void prof_begin(int fam);
     void prof_end(int fam);
     void prof_collect(bool wait_all);
-    void launch_face_groups(TileList tl, int grid, bool full, int stage);
+    void launch_face_iter_groups(TileList tl, int grid, int stage);
+    int face_variant(int R, int stage);
     template <class T>
     int grow(T** p, size_t& cap, size_t need, bool host, unsigned host_flags = hipHostMallocDefault);
 
@@ -139,6 +155,12 @@ class Engine {
     size_t cap_hdt_ = 0;
     Item* d_pad_ = nullptr;
     size_t cap_pad_ = 0;
+    // every active solve's first-pass face tiles of each shape, and their
+    // number (for a single solve, its own tiles follow from the block index)
+    Item* d_fitems_[3] = {nullptr, nullptr, nullptr};
+    size_t cap_fitems_[3] = {0, 0, 0};
+    int nfitems_[3] = {0, 0, 0};
+    long long rcells_[kMaxR + 1] = {0};  // padded cells of the active solves of each radius
     Item* d_cell_ = nullptr;
     size_t cap_cell_ = 0;
     Item* d_act_ = nullptr;  // the two active lists

@@ -38,7 +38,8 @@ EXPORTS = (
     "hmhd_set_gp",
     "hmhd_upload_state", "hmhd_cfl_dt", "hmhd_advance", "hmhd_download", "hmhd_stats",
     "hmhd_record_stage_schemes", "hmhd_stage_schemes", "hmhd_profile", "hmhd_profile_read",
-    "hmhd_synchronize", "hmhd_cfl_dt_noalloc",
+    "hmhd_synchronize", "hmhd_cfl_dt_noalloc", "hmhd_face_tuning", "hmhd_face_tuning_read",
+    "hmhd_face_tuning_clear",
     "hmhd_batch_create", "hmhd_batch_destroy", "hmhd_batch_last_error", "hmhd_batch_add", "hmhd_batch_run",
     "hmhd_batch_status", "hmhd_batch_dims", "hmhd_batch_download", "hmhd_batch_profile",
     "hmhd_batch_profile_read",
@@ -66,6 +67,11 @@ class _Config(C.Structure):
 class _Result(C.Structure):
     _fields_ = [("state", C.c_int32), ("steps", C.c_int32), ("t", C.c_double), ("zone_updates", C.c_int64),
                 ("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
+class _FaceTune(C.Structure):
+    _fields_ = [("R", C.c_int32), ("size_class", C.c_int32), ("form", C.c_int32), ("tile_x", C.c_int32),
+                ("tile_y", C.c_int32), ("waves", C.c_int32), ("ms", C.c_double * 8)]
 
 
 class _ServerStats(C.Structure):
@@ -134,6 +140,9 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.hmhd_profile_read.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int32]
     lib.hmhd_synchronize.argtypes = [vp]
     lib.hmhd_cfl_dt_noalloc.argtypes = [C.POINTER(_Config), vp, vp, vp, vp]
+    lib.hmhd_face_tuning.argtypes = [C.c_int32]
+    lib.hmhd_face_tuning_read.argtypes = [C.POINTER(_FaceTune), C.c_int32, C.POINTER(C.c_int32)]
+    lib.hmhd_face_tuning_clear.argtypes = []
     i64, i64p = C.c_int64, C.POINTER(C.c_int64)
     lib.hmhd_batch_create.argtypes = [C.POINTER(vp)]
     lib.hmhd_batch_destroy.argtypes = [vp]
@@ -315,6 +324,40 @@ def cfl_dt_noalloc(cfg: Config, U_ptr: int, V_ptr: int | None, dt_ptr: int, stre
                                  C.c_void_p(dt_ptr), C.c_void_p(stream) if stream else None)
     if rc != 0:
         raise HmhdError(rc, lib.hmhd_last_error(None).decode())
+
+
+FACE_FORMS = ((16, 16, 2), (32, 8, 2), (32, 16, 2), (16, 16, 1), (32, 8, 1))
+
+
+def face_tuning(mode: int | str = "tune") -> None:
+    """The first-pass face kernel's form: "tune" (measure per GP radius and
+    launch size class, keep the fastest; the default), "default" (16 x 16,
+    two waves) or a form index into FACE_FORMS."""
+    m = 1 if mode == "tune" else 0 if mode == "default" else 2 + int(mode)
+    lib = load_library()
+    rc = lib.hmhd_face_tuning(m)
+    if rc != 0:
+        raise HmhdError(rc, lib.hmhd_last_error(None).decode())
+
+
+def face_tuning_table() -> list[dict]:
+    """The kept forms: R, size class, form, its tile and waves, and the
+    measured milliseconds per form."""
+    lib = load_library()
+    n = C.c_int32()
+    buf = (_FaceTune * 64)()
+    lib.hmhd_face_tuning_read(buf, 64, C.byref(n))
+    out = []
+    for e in buf[:min(n.value, 64)]:
+        out.append({"R": e.R, "size_class": e.size_class, "form": e.form, "tile": [e.tile_x, e.tile_y],
+                    "waves": e.waves,
+                    "ms_per_form": {f"{FACE_FORMS[v][0]}x{FACE_FORMS[v][1]}/w{FACE_FORMS[v][2]}": e.ms[v]
+                                    for v in range(len(FACE_FORMS)) if e.ms[v] > 0}})
+    return out
+
+
+def face_tuning_clear() -> None:
+    load_library().hmhd_face_tuning_clear()
 
 
 def _padded(cfg: Config):

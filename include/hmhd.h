@@ -132,6 +132,28 @@ int hmhd_profile_read(hmhd_t* h, double* ms, int64_t* launches, int32_t reset);
 /* Wait for all work enqueued on the handle. */
 int hmhd_synchronize(hmhd_t* h);
 
+/* The first pass's face kernel (the dominant one) comes in tuned forms:
+ * form v = 0..HMHD_FACE_FORMS-1 is a tile of 16 x 16, 32 x 8 or 32 x 16
+ * cells at two waves per SIMD (v = 0, 1, 2) or 16 x 16, 32 x 8 at one
+ * (v = 3, 4); GP radius 3 runs at one wave only.  Every form computes the
+ * same bits.  Process-wide, for every handle, batch and server:
+ * mode 1 (the default) measures every form on the first first pass of each
+ * (radius, launch size class) and keeps the fastest; mode 0 always runs
+ * form 0; mode 2 + v always runs form v.  No counterpart in the reference
+ * (build-owned tuning). */
+#define HMHD_FACE_FORMS 5
+typedef struct hmhd_face_tune {
+    int32_t R;
+    int32_t size_class;              /* padded cells of the launch: < 2^16, < 2^19, < 2^21, or more */
+    int32_t form;                    /* the form kept */
+    int32_t tile_x, tile_y, waves;   /* its shape and wave target */
+    double ms[8];                    /* per form: fastest of three launches, milliseconds (0: not run) */
+} hmhd_face_tune;
+int hmhd_face_tuning(int32_t mode);
+/* Copies up to max entries of the tuning table; *n receives their number. */
+int hmhd_face_tuning_read(hmhd_face_tune* out, int32_t max, int32_t* n);
+int hmhd_face_tuning_clear(void);
+
 /* No counterpart in the reference.  For callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc): the dt of hmhd_cfl_dt (the same

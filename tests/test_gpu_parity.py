@@ -260,3 +260,31 @@ def test_gpu_reproduces_reference_fixture(name):
     assert np.array_equal(Ug, fx["U"]) and np.array_equal(Vg, fx["V"])
     assert np.array_equal(gpu.stage_schemes().astype(np.int8), fx["stage_schemes"])
     assert np.array_equal(gpu.stats()[0], fx["passes"])
+
+
+@pytest.mark.parametrize("form", range(len(rt.FACE_FORMS)))
+def test_every_face_kernel_form_is_bitwise(form):
+    """Each tuned form of the first pass's face kernel (tile shape and wave
+    target, forced) reproduces the oracle bit for bit: R = 2 Orszag-Tang on
+    a grid that is not a multiple of any tile, outflow blast, and R = 3."""
+    try:
+        rt.face_tuning(form)
+        for name, n, R in (("ot", 50, 2), ("blast", 40, 2), ("ot_ref", 36, 3)):
+            cfg, g, orc, gpu = _pair(name, n=n, R=R)
+            for k in range(3):
+                _step_both(orc, gpu)
+                _compare_state(orc, gpu, f"form {form}, {name} {n}^2 R={R}, step {k + 1}")
+            gpu.close()
+    finally:
+        rt.face_tuning("tune")
+
+
+def test_face_tuning_table_records_every_form():
+    rt.face_tuning_clear()
+    cfg, g, orc, gpu = _pair("ot", n=64)
+    _step_both(orc, gpu)
+    _compare_state(orc, gpu, "after tuning")
+    table = rt.face_tuning_table()
+    e = [x for x in table if x["R"] == 2]
+    assert e and len(e[0]["ms_per_form"]) == len(rt.FACE_FORMS)
+    gpu.close()
