@@ -57,25 +57,46 @@ UNIT = "zone-updates/s"
 PEAK_FP64_TFLOPS = 78.6  # FP64 vector == FP64 matrix on CDNA4
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic FP64 flops (+ - * / sqrt = 1; min/max/abs/compare = 0) of the
-# first MOOD pass at GP radius 2, counted from the arithmetic each kernel
-# performs, summed per zone below.
-FLOP_CONS2PRIM = 21
-FLOP_CONS2PRES = 17
-FLOP_FAST_SPEED = 21
-FLOP_CONS2FLUX = 33      # given the primitives
-FLOP_HLLC_STAR = 2 * FLOP_CONS2PRIM + 20 + 3 * FLOP_FAST_SPEED + 4 + 2 * FLOP_CONS2FLUX + 49 + 18 + 14 + 2 + 13 + 8 + 27 + 24
-FLOP_RECON_R2 = 4 * 3 * 13 * 8 * 2      # 4 face sides x 3 points x 13 cells x 8 vars, mul + add
-FLOP_ADMISSIBILITY_R2 = 4 * 3 * FLOP_CONS2PRES
-FLOP_FLUX_SUM_R2 = 2 * 3 * 16           # 2 owned faces x 3 points x (8 mul + 8 add)
-FLOP_FACE_KERNEL_R2 = FLOP_RECON_R2 + FLOP_ADMISSIBILITY_R2 + 2 * 3 * FLOP_HLLC_STAR + FLOP_FLUX_SUM_R2
-FLOP_UPDATE_KERNEL = 2 * 8 + 8 * 6 + (FLOP_CONS2PRES + 5 * FLOP_CONS2PRIM + 7 + 7 + 5)
-FLOP_RK_COMBINE = 2 * 8 * 3
-FLOP_CFL = FLOP_CONS2PRIM + 5 + 16 + 2
-FLOP_ZONE_UPDATE_R2 = 3 * (FLOP_FACE_KERNEL_R2 + FLOP_UPDATE_KERNEL) + FLOP_RK_COMBINE + FLOP_CFL
+# Algorithmic FP64 flops come from hmhd_flop_count (built with the library):
+# the kernels' own formulas compiled on the host with a counting scalar
+# (+ - * / sqrt = 1; min/max/abs/compare = 0), run over the solve's states.
+FLOP_COUNT_EXE = PKG_DIR / "hmhd_flop_count"
 # compulsory HBM bytes of the face kernel per interior cell: read U (8 FP64),
 # write the raw x and y fluxes (16 FP64) and the two FLICD components (2 FP64)
 BYTES_FACE_KERNEL = 8 * 8 + 16 * 8 + 2 * 8
+
+
+def count_flops(cfg, states, windows: int = 64) -> dict | None:
+    """Instrumented FP64 operation counts of the first MOOD pass over each
+    padded state U (my, mx, 8) in `states` (hmhd_flop_count: face and update
+    work averaged over `windows` seeded 32 x 32 windows, the face kernel's
+    domain outside the interior in full), averaged over the states."""
+    import numpy as np
+    import tempfile
+    if not FLOP_COUNT_EXE.exists():
+        return None
+    outs = []
+    with tempfile.TemporaryDirectory() as d:
+        for k, U in enumerate(states):
+            path = pathlib.Path(d) / f"U{k}.bin"
+            np.ascontiguousarray(U, dtype=np.float64).tofile(path)
+            args = [str(FLOP_COUNT_EXE), str(path), str(cfg.nx), str(cfg.ny), str(cfg.R), repr(cfg.ell_over_delta),
+                    repr(cfg.gamma), repr(cfg.small_p), repr(cfg.small_rho), str(int(cfg.force_hydro)), str(cfg.bc),
+                    repr(cfg.xbeg), repr(cfg.xend), repr(cfg.ybeg), repr(cfg.yend), repr(cfg.cfl), str(windows),
+                    str(k + 1)]
+            r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+            if r.returncode != 0:
+                return {"error": r.stderr.strip()[-300:]}
+            outs.append(json.loads(r.stdout))
+    mean = lambda key: sum(o[key] for o in outs) / len(outs)
+    per_zone = {k: sum(o["per_zone"][k] for o in outs) / len(outs) for k in outs[0]["per_zone"]}
+    return {"flops_per_zone_update": mean("flops_per_zone_update"),
+            "face_flops_per_launch": mean("face_flops_per_launch"),
+            "hllc_flops_per_call": mean("hllc_flops_per_call"), "per_zone": per_zone,
+            "states": len(outs), "windows_per_state": windows,
+            "how": "hmhd_flop_count: the kernels' formulas (hmhd_math.hpp) on the host with a counting FP64 "
+                   "type; first MOOD pass of a stage (every cell at radius R), x3 stages, plus the RK "
+                   "combinations and the CFL pass; + - * / sqrt count 1, min/max/abs/compare 0"}
 
 
 def load_package():
@@ -412,7 +433,9 @@ def plain_form(rt, args, cpu_proc=None):
     s.profile_read(reset=True)
     tmax, K, zones = args.tmax, args.steps, args.n * args.n
     samp_t = [(k + 0.5) * tmax / K for k in range(K)] if K > 0 else []
-    cpu_t = [(j + 0.5) * tmax / 4 for j in range(4)] if cpu_proc is not None else []
+    # states at t = (j + 1/2) t_end / 4: the CPU baseline's start states and
+    # the operation count's inputs
+    cpu_t = [(j + 0.5) * tmax / 4 for j in range(4)]
     si = ci = 0
     state_paths = []
     samp_s, samp_n, samp_passes, samp_first = 0.0, 0, 0, []
@@ -474,6 +497,7 @@ def plain_form(rt, args, cpu_proc=None):
     prof = s.profile_read()
     s.close()
     cpu = finish_cpu_baseline(cpu_proc, [q for q, _ in state_paths]) if cpu_proc is not None else None
+    flops = count_flops(cfg, [np.load(q)["U"] for q, _ in state_paths] or [U])
     for q, _ in state_paths:
         q.unlink(missing_ok=True)
 
@@ -492,7 +516,9 @@ def plain_form(rt, args, cpu_proc=None):
 
     ms_full, n_full = prof["face_full"]
     avg_s = ms_full / n_full / 1e3 if n_full else float("nan")
-    flops_launch = zones * FLOP_FACE_KERNEL_R2
+    if not flops or "error" in flops:
+        raise RuntimeError(f"hmhd_flop_count failed or is not built: {flops}")
+    flops_launch = flops["face_flops_per_launch"]
     achieved = flops_launch / avg_s / 1e12
     traffic = None
     tf = ROOT / "profiles" / "face_kernel_traffic_ot1024.json"
@@ -518,7 +544,9 @@ def plain_form(rt, args, cpu_proc=None):
         "timed_sample_steps": sampled["steps"],
         "full_solve": full,
         "roofline": roofline,
-        "flops_per_zone_update": FLOP_ZONE_UPDATE_R2,
+        "flops_per_zone_update": flops["flops_per_zone_update"],
+        "flop_count": flops,
+        "fp64_tflops_full_solve": flops["flops_per_zone_update"] * full["value"] / 1e12,
         "kernel_breakdown": {"steps": bd_range,
                              "ms": {k: v[0] for k, v in breakdown.items()},
                              "launches": {k: v[1] for k, v in breakdown.items()},

@@ -37,25 +37,6 @@ namespace hmhd {
 __constant__ double c_pv[kMaxGpSets][kPvSet];
 __constant__ double c_qw[kMaxGpSets][kMaxR + 1][4];
 
-// ------------------------------------------------------------------ //
-// stencil of radius RR in the reference's order: i outer, j inner,
-// |i|+|j| <= RR (src/GP.f90:85-95)
-template <int RR>
-struct Sten {
-    static constexpr int n = gp_ns(RR);
-    int x[n], y[n];
-    constexpr Sten() : x(), y() {
-        int c = 0;
-        for (int i = -RR; i <= RR; i++)
-            for (int j = -RR; j <= RR; j++)
-                if ((i < 0 ? -i : i) + (j < 0 ? -j : j) <= RR) {
-                    x[c] = i;
-                    y[c] = j;
-                    c++;
-                }
-    }
-};
-
 // padded index -> interior index along one axis (guard fill as index map)
 __device__ __forceinline__ int map_axis(int p, int ngc, int n, int bc) {
     int k = p - ngc;
@@ -75,21 +56,6 @@ __device__ __forceinline__ Phys phys_of(const Inst& I) {
     ph.small_rho = I.small_rho;
     ph.hydro = I.hydro;
     return ph;
-}
-
-// GP point value at quadrature point q of face f of the cell whose stencil
-// the accessor A is centred on; radius RR (src/reconstruct.f90:109-124)
-template <int RR, class A>
-__device__ __forceinline__ void gp_point(const A& a, const double* __restrict__ w, double* out) {
-    constexpr Sten<RR> st{};
-#pragma unroll
-    for (int v = 0; v < NCONS; v++) out[v] = 0.0;
-#pragma unroll
-    for (int k = 0; k < st.n; k++) {
-        const double wk = w[k];
-#pragma unroll
-        for (int v = 0; v < NCONS; v++) out[v] = out[v] + wk * a.at(v, st.x[k], st.y[k]);
-    }
 }
 
 template <int LX, int LP>
@@ -944,30 +910,6 @@ __device__ __forceinline__ void sync_block(const Inst& I, int bx, int gx) {
 // stage 2: out = (base + 2 Unew)/3.
 
 
-// NaN-skipping MINVAL / MAXVAL of five values
-__device__ __forceinline__ double minval5(const double* x) {
-    double m = x[0];
-#pragma unroll
-    for (int k = 1; k < 5; k++)
-        if (__builtin_isnan(m) || x[k] < m) m = x[k];
-    return m;
-}
-__device__ __forceinline__ double maxval5(const double* x) {
-    double m = x[0];
-#pragma unroll
-    for (int k = 1; k < 5; k++)
-        if (__builtin_isnan(m) || x[k] > m) m = x[k];
-    return m;
-}
-
-// The values around one cell that detection reads, loaded together
-// before any is used: U of the cross neighbours and the density of the
-// cells at taxicab distance 2 (guard cells through the index map).
-struct Around {
-    double e[NCONS], w[NCONS], n[NCONS], s[NCONS];  // U at (i+1,j), (i-1,j), (i,j+1), (i,j-1)
-    double d2[8];  // rho at (i+2,j), (i-2,j), (i,j+2), (i,j-2), (i+1,j+1), (i+1,j-1), (i-1,j+1), (i-1,j-1)
-};
-
 __device__ __forceinline__ void load_around(const Inst& I, const double* __restrict__ U, int i, int j, Around& a) {
     const size_t plane = (size_t)I.nx * I.ny;
     auto off = [&](int di, int dj) {
@@ -989,72 +931,6 @@ __device__ __forceinline__ void load_around(const Inst& I, const double* __restr
     a.d2[5] = U[off(1, -1)];
     a.d2[6] = U[off(-1, 1)];
     a.d2[7] = U[off(-1, -1)];
-}
-
-// density at offset (di, dj), |di| + |dj| <= 2, from the centre cell and `a`
-__device__ __forceinline__ double dens_at(const double* uc, const Around& a, int di, int dj) {
-    if (di == 0 && dj == 0) return uc[DENS];
-    if (di == 1 && dj == 0) return a.e[DENS];
-    if (di == -1 && dj == 0) return a.w[DENS];
-    if (di == 0 && dj == 1) return a.n[DENS];
-    if (di == 0 && dj == -1) return a.s[DENS];
-    if (di == 2) return a.d2[0];
-    if (di == -2) return a.d2[1];
-    if (dj == 2) return a.d2[2];
-    if (dj == -2) return a.d2[3];
-    if (di == 1) return dj == 1 ? a.d2[4] : a.d2[5];
-    return dj == 1 ? a.d2[6] : a.d2[7];
-}
-
-// src/detect.f90:83-213 for one checked cell; true = lower the order
-__device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const Around& a, const double* un,
-                                            const double* uc) {
-    const double presNew = cons2pres(ph, un);
-    if (!admissible(un[DENS], presNew)) return true;
-    const Prim V = cons2prim(ph, uc);
-    const Prim Vip = cons2prim(ph, a.e);
-    const Prim Vim = cons2prim(ph, a.w);
-    const Prim Vjp = cons2prim(ph, a.n);
-    const Prim Vjm = cons2prim(ph, a.s);
-    double div_vel = (Vip.vx - Vim.vx) / (2.0 * I.dx);
-    div_vel = div_vel + (Vjp.vy - Vjm.vy) / (2.0 * I.dy);
-    const bool shock1 = div_vel < -(1.0 * (I.maxdxdy * I.maxdxdy));
-    double mach = V.vx * V.vx + V.vy * V.vy;
-    mach = V.d * mach / (ph.gamma * V.p);
-    mach = __builtin_sqrt(mach);
-    const bool shock2 = mach > 0.2;
-    double pg = __builtin_fabs(Vip.p - Vim.p) / fminF(Vip.p, Vim.p);
-    pg = pg + __builtin_fabs(Vjp.p - Vjm.p) / fminF(Vjp.p, Vjm.p);
-    const bool shock3 = pg > 10.0 * I.maxdxdy;
-    if (!(shock1 && shock2 && shock3)) return false;
-
-    const double densNew = un[DENS];
-    const double d1 = uc[DENS], d2 = a.e[DENS], d3 = a.w[DENS], d4 = a.n[DENS], d5 = a.s[DENS];
-    const double mn = fminF(fminF(fminF(fminF(d1, d2), d3), d4), d5);
-    const double mxd = fmaxF(fmaxF(fmaxF(fmaxF(d1, d2), d3), d4), d5);
-    if (mxd - mn < I.mindxdy * I.mindxdy * I.mindxdy) return false;
-    if ((mn <= densNew) && (densNew <= mxd)) return false;
-
-    // u2 smoothness on the five cells of the cross (src/detect.f90:177-213)
-    double ddx[5], ddy[5];
-    const double dx2 = I.dx * I.dx, dy2 = I.dy * I.dy;
-    constexpr int ci[5] = {-1, 0, 0, 0, 1}, cj[5] = {0, -1, 0, 1, 0};
-#pragma unroll
-    for (int c = 0; c < 5; c++) {
-        const double dcc = dens_at(uc, a, ci[c], cj[c]);
-        ddx[c] = (dens_at(uc, a, ci[c] + 1, cj[c]) - 2.0 * dcc + dens_at(uc, a, ci[c] - 1, cj[c])) / dx2;
-        ddy[c] = (dens_at(uc, a, ci[c], cj[c] + 1) - 2.0 * dcc + dens_at(uc, a, ci[c], cj[c] - 1)) / dy2;
-    }
-    const double mnx = minval5(ddx), mny = minval5(ddy), mxx = maxval5(ddx), mxy = maxval5(ddy);
-    const double m = I.mindxdy;
-    const bool sx1 = mnx * mxx > -m, sy1 = mny * mxy > -m;
-    const double ax = fmaxF(__builtin_fabs(mnx), __builtin_fabs(mxx)), ay = fmaxF(__builtin_fabs(mny), __builtin_fabs(mxy));
-    const bool sx2 = ax < m, sy2 = ay < m;
-    const bool sx3 = fminF(__builtin_fabs(mnx), __builtin_fabs(mxx)) >= 0.5 * ax;
-    const bool sy3 = fminF(__builtin_fabs(mny), __builtin_fabs(mxy)) >= 0.5 * ay;
-    const bool smooth1 = sx1 && sx2 && sy1 && sy2;
-    const bool smooth2 = sx1 && sx3 && sy1 && sy3;
-    return !(smooth1 || smooth2);
 }
 
 // One tile of the FV update.  An own (rcF) cell loads everything its
@@ -1109,14 +985,14 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     }
     load_around(I, Uin, i, j, a);
     // src/FLICDdivB.f90:38-45 and :47-54 at (i, j)
-    const double cx = (2.0 * x0 + x1 + xw - y1 - y0 - yw0 - yw1) / 8.0;
-    const double cy = (2.0 * y0 + y1 + ys - x1 - x0 - xs0 - xs1) / 8.0;
+    const double cx = flicd_corr(x0, x1, xw, y1, y0, yw0, yw1);
+    const double cy = flicd_corr(y0, y1, ys, x1, x0, xs0, xs1);
     I.FLB[idx] = cx;
     I.FLB[fp + idx] = cy;
     if (!inner) return;
     // the east / north neighbour's correction, or the value its face holds
-    const double cxe = rcfE ? (2.0 * x1 + x2 + x0 - ye1 - ye0 - y0 - y1) / 8.0 : flbE;
-    const double cyn = rcfN ? (2.0 * y1 + y2 + y0 - xn1 - xn0 - x0 - x1) / 8.0 : flbN;
+    const double cxe = rcfE ? flicd_corr(x1, x2, x0, ye1, ye0, y0, y1) : flbE;
+    const double cyn = rcfN ? flicd_corr(y1, y2, y0, xn1, xn0, x0, x1) : flbN;
     fxm[MAGX] = 0.0;
     fxp[MAGX] = 0.0;
     fym[MAGX] = cy;
@@ -1128,7 +1004,7 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     const double dtdx = I.dtdx, dtdy = I.dtdy;
     double un[NCONS];
 #pragma unroll
-    for (int v = 0; v < NCONS; v++) un[v] = uc[v] - dtdx * (fxp[v] - fxm[v]) - dtdy * (fyp[v] - fym[v]);
+    for (int v = 0; v < NCONS; v++) un[v] = fv_update(uc[v], dtdx, fxp[v], fxm[v], dtdy, fyp[v], fym[v]);
     bool lower = false;
     if (sch > 0) {
         const Phys ph = phys_of(I);
@@ -1137,16 +1013,8 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     I.low[idx] = lower ? 1 : 0;
     if (lower) I.ctl[CTL_ANY_LOWER] = 1;
     double* __restrict__ Uout = ubuf(I, sb.out);
-    if (stage == 0) {
 #pragma unroll
-        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = un[v];
-    } else if (stage == 1) {
-#pragma unroll
-        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (3.0 * ub[v] + un[v]) / 4.0;
-    } else {
-#pragma unroll
-        for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = (ub[v] + 2.0 * un[v]) / 3.0;
-    }
+    for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = rk_combine(stage, ub[v], un[v]);
 }
 
 template <bool ONE>

@@ -12,9 +12,13 @@
 //   CAD / PAD               src/detect.f90:238-265
 // MAX(a,b) is a>b?a:b and MIN(a,b) is a<b?a:b, the rule the reference's
 // compiler applies.
+// The file also compiles on the host with HMHD_MATH_HOST defined (the
+// instrumented operation count, flop_count.cpp, runs these same formulas).
 #pragma once
 
+#ifndef HMHD_MATH_HOST
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #include "hmhd_internal.hpp"
@@ -246,6 +250,150 @@ __device__ __forceinline__ void hllc(const Phys& ph, const double* uL, const dou
 #pragma unroll
         for (int v = 0; v < NCONS; v++) flux[v] = fR[v];
     }
+}
+
+
+// ------------------------------------------------------------------ //
+// Reconstruction, FV update and detection formulas of the per-cell kernels.
+
+// stencil of radius RR in the reference's order: i outer, j inner,
+// |i|+|j| <= RR (src/GP.f90:85-95)
+template <int RR>
+struct Sten {
+    static constexpr int n = gp_ns(RR);
+    int x[n], y[n];
+    constexpr Sten() : x(), y() {
+        int c = 0;
+        for (int i = -RR; i <= RR; i++)
+            for (int j = -RR; j <= RR; j++)
+                if ((i < 0 ? -i : i) + (j < 0 ? -j : j) <= RR) {
+                    x[c] = i;
+                    y[c] = j;
+                    c++;
+                }
+    }
+};
+
+// GP point value at quadrature point q of face f of the cell whose stencil
+// the accessor A is centred on; radius RR (src/reconstruct.f90:109-124)
+template <int RR, class A>
+__device__ __forceinline__ void gp_point(const A& a, const double* __restrict__ w, double* out) {
+    constexpr Sten<RR> st{};
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) out[v] = 0.0;
+#pragma unroll
+    for (int k = 0; k < st.n; k++) {
+        const double wk = w[k];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) out[v] = out[v] + wk * a.at(v, st.x[k], st.y[k]);
+    }
+}
+
+// FLICD correction (src/FLICDdivB.f90:38-54): (2 a + b + c - d - e - f - g) / 8,
+// summed left to right
+__device__ __forceinline__ double flicd_corr(double a, double b, double c, double d, double e, double f, double g) {
+    return (2.0 * a + b + c - d - e - f - g) / 8.0;
+}
+
+// the unsplit FV update of one variable (src/timeStep.f90:112-114)
+__device__ __forceinline__ double fv_update(double uc, double dtdx, double fxp, double fxm, double dtdy, double fyp,
+                                            double fym) {
+    return uc - dtdx * (fxp - fxm) - dtdy * (fyp - fym);
+}
+
+// SSP-RK3 combination of stage `stage` (src/timeStep.f90:165,169)
+__device__ __forceinline__ double rk_combine(int stage, double ub, double un) {
+    return stage == 0 ? un : (stage == 1 ? (3.0 * ub + un) / 4.0 : (ub + 2.0 * un) / 3.0);
+}
+
+// NaN-skipping MINVAL / MAXVAL of five values
+__device__ __forceinline__ double minval5(const double* x) {
+    double m = x[0];
+#pragma unroll
+    for (int k = 1; k < 5; k++)
+        if (__builtin_isnan(m) || x[k] < m) m = x[k];
+    return m;
+}
+__device__ __forceinline__ double maxval5(const double* x) {
+    double m = x[0];
+#pragma unroll
+    for (int k = 1; k < 5; k++)
+        if (__builtin_isnan(m) || x[k] > m) m = x[k];
+    return m;
+}
+
+// The values around one cell that detection reads, loaded together
+// before any is used: U of the cross neighbours and the density of the
+// cells at taxicab distance 2 (guard cells through the index map).
+struct Around {
+    double e[NCONS], w[NCONS], n[NCONS], s[NCONS];  // U at (i+1,j), (i-1,j), (i,j+1), (i,j-1)
+    double d2[8];  // rho at (i+2,j), (i-2,j), (i,j+2), (i,j-2), (i+1,j+1), (i+1,j-1), (i-1,j+1), (i-1,j-1)
+};
+
+// density at offset (di, dj), |di| + |dj| <= 2, from the centre cell and `a`
+__device__ __forceinline__ double dens_at(const double* uc, const Around& a, int di, int dj) {
+    if (di == 0 && dj == 0) return uc[DENS];
+    if (di == 1 && dj == 0) return a.e[DENS];
+    if (di == -1 && dj == 0) return a.w[DENS];
+    if (di == 0 && dj == 1) return a.n[DENS];
+    if (di == 0 && dj == -1) return a.s[DENS];
+    if (di == 2) return a.d2[0];
+    if (di == -2) return a.d2[1];
+    if (dj == 2) return a.d2[2];
+    if (dj == -2) return a.d2[3];
+    if (di == 1) return dj == 1 ? a.d2[4] : a.d2[5];
+    return dj == 1 ? a.d2[6] : a.d2[7];
+}
+
+// src/detect.f90:83-213 for one checked cell; true = lower the order
+__device__ __forceinline__ bool detect_cell(const Inst& I, const Phys& ph, const Around& a, const double* un,
+                                            const double* uc) {
+    const double presNew = cons2pres(ph, un);
+    if (!admissible(un[DENS], presNew)) return true;
+    const Prim V = cons2prim(ph, uc);
+    const Prim Vip = cons2prim(ph, a.e);
+    const Prim Vim = cons2prim(ph, a.w);
+    const Prim Vjp = cons2prim(ph, a.n);
+    const Prim Vjm = cons2prim(ph, a.s);
+    double div_vel = (Vip.vx - Vim.vx) / (2.0 * I.dx);
+    div_vel = div_vel + (Vjp.vy - Vjm.vy) / (2.0 * I.dy);
+    const bool shock1 = div_vel < -(1.0 * (I.maxdxdy * I.maxdxdy));
+    double mach = V.vx * V.vx + V.vy * V.vy;
+    mach = V.d * mach / (ph.gamma * V.p);
+    mach = __builtin_sqrt(mach);
+    const bool shock2 = mach > 0.2;
+    double pg = __builtin_fabs(Vip.p - Vim.p) / fminF(Vip.p, Vim.p);
+    pg = pg + __builtin_fabs(Vjp.p - Vjm.p) / fminF(Vjp.p, Vjm.p);
+    const bool shock3 = pg > 10.0 * I.maxdxdy;
+    if (!(shock1 && shock2 && shock3)) return false;
+
+    const double densNew = un[DENS];
+    const double d1 = uc[DENS], d2 = a.e[DENS], d3 = a.w[DENS], d4 = a.n[DENS], d5 = a.s[DENS];
+    const double mn = fminF(fminF(fminF(fminF(d1, d2), d3), d4), d5);
+    const double mxd = fmaxF(fmaxF(fmaxF(fmaxF(d1, d2), d3), d4), d5);
+    if (mxd - mn < I.mindxdy * I.mindxdy * I.mindxdy) return false;
+    if ((mn <= densNew) && (densNew <= mxd)) return false;
+
+    // u2 smoothness on the five cells of the cross (src/detect.f90:177-213)
+    double ddx[5], ddy[5];
+    const double dx2 = I.dx * I.dx, dy2 = I.dy * I.dy;
+    constexpr int ci[5] = {-1, 0, 0, 0, 1}, cj[5] = {0, -1, 0, 1, 0};
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        const double dcc = dens_at(uc, a, ci[c], cj[c]);
+        ddx[c] = (dens_at(uc, a, ci[c] + 1, cj[c]) - 2.0 * dcc + dens_at(uc, a, ci[c] - 1, cj[c])) / dx2;
+        ddy[c] = (dens_at(uc, a, ci[c], cj[c] + 1) - 2.0 * dcc + dens_at(uc, a, ci[c], cj[c] - 1)) / dy2;
+    }
+    const double mnx = minval5(ddx), mny = minval5(ddy), mxx = maxval5(ddx), mxy = maxval5(ddy);
+    const double m = I.mindxdy;
+    const bool sx1 = mnx * mxx > -m, sy1 = mny * mxy > -m;
+    const double ax = fmaxF(__builtin_fabs(mnx), __builtin_fabs(mxx)), ay = fmaxF(__builtin_fabs(mny), __builtin_fabs(mxy));
+    const bool sx2 = ax < m, sy2 = ay < m;
+    const bool sx3 = fminF(__builtin_fabs(mnx), __builtin_fabs(mxx)) >= 0.5 * ax;
+    const bool sy3 = fminF(__builtin_fabs(mny), __builtin_fabs(mxy)) >= 0.5 * ay;
+    const bool smooth1 = sx1 && sx2 && sy1 && sy2;
+    const bool smooth2 = sx1 && sx3 && sy1 && sy3;
+    return !(smooth1 || smooth2);
 }
 
 }  // namespace hmhd
