@@ -261,7 +261,9 @@ class RefHarness:
 
     _libs: dict = {}
 
-    def __init__(self, cfg: Config, variant: str = "exact"):
+    def __init__(self, cfg: Config | None, variant: str = "exact", par_path=None):
+        """Set up the reference's grid from cfg, or, with par_path, from that
+        .par file as the reference's driver reads it (cfg may be None)."""
         self.cfg = cfg
         path = HERE / "_ref" / f"libref_{variant}.so"
         if variant not in RefHarness._libs:
@@ -279,11 +281,16 @@ class RefHarness:
             lib.ref_reconstruct_all.argtypes = [C.c_int, dp]
             lib.ref_step_masks.argtypes = [C.c_double, dp, ip, ip]
             lib.ref_flush.argtypes = []
+            lib.ref_initial_condition.argtypes = [C.c_char_p]
             RefHarness._libs[variant] = lib
         self.lib = RefHarness._libs[variant]
         self._tmp = tempfile.TemporaryDirectory()
-        par = pathlib.Path(self._tmp.name) / "run.par"
-        par.write_text(cfg.par_text())
+        if par_path is None:
+            par = pathlib.Path(self._tmp.name) / "run.par"
+            par.write_text(cfg.par_text())
+        else:
+            par = pathlib.Path(par_path)
+        self.par = par
         # the reference prints banners on stdout through the Fortran runtime
         self.lib.ref_init(str(par).encode())
         d = [C.c_int() for _ in range(6)]
@@ -292,6 +299,13 @@ class RefHarness:
 
     def flush(self) -> None:
         self.lib.ref_flush()
+
+    def initial_condition(self):
+        """src/initialCondition.f90:23-84 from this harness's .par file;
+        returns (U, V) with guard cells, as get_state."""
+        self.lib.ref_initial_condition(str(self.par).encode())
+        U, V, _ = self.get_state()
+        return U, V
 
     def gp_tables(self):
         pv = np.zeros((self.rmax, 4, self.nqmax, self.nsmax))

@@ -2,7 +2,8 @@
 ! modules, compiled from the reference sources by oracle/Makefile into
 ! oracle/_ref/.  TEST INFRASTRUCTURE ONLY: it lets tests/ and bench.py's
 ! cpu_baseline leg run the reference's hot path (src/cfl.f90:14-52 and
-! src/timeStep.f90:22-171) on inputs the tests choose.  Nothing in the
+! src/timeStep.f90:22-171) on inputs the tests choose, and set the
+! reference's initial conditions (src/initialCondition.f90:23-84).  Nothing in the
 ! product library links or calls it.
 !
 ! Layouts match the reference: U(8, 1:mx, 1:my), V(10, 1:mx, 1:my),
@@ -22,6 +23,7 @@ module ref_harness
     use FLICDdivB, only: FLICDdivB_fluxCorrections
     use detect, only: detect_ALL
     use boundaryConditions, only: boundaryConditions_applyBlk, boundaryConditions_applyMOOD
+    use initialCondition, only: initialCondition_set
 
     implicit none
 
@@ -52,6 +54,15 @@ contains
         call grid_init(pf, GP_maxRadius, GP_nQuadratureMax)
         ref_ready = .true.
     end subroutine ref_init
+
+    ! src/initialCondition.f90:23-84 on the grid ref_init set up: IC_type
+    ! and its IC_* parameters read from the same .par file
+    subroutine ref_initial_condition(parfile) bind(C, name='ref_initial_condition')
+        character(kind=c_char), intent(in) :: parfile(*)
+        character(len=max_string_length) :: pf
+        call c2f_string(parfile, pf)
+        call initialCondition_set(pf, grid_block, grid_state)
+    end subroutine ref_initial_condition
 
     subroutine ref_finalize() bind(C, name='ref_finalize')
         if (ref_ready) then

@@ -107,6 +107,111 @@ def test_initial_conditions_match_the_python_restatement(tmp_path, ic_type, extr
     magp = (V[js, is_, 5] ** 2 + V[js, is_, 6] ** 2 + V[js, is_, 7] ** 2) / 2
     assert np.array_equal(d["magp"], magp)
     assert np.array_equal(d["ener"], ener + magp)
+    assert np.array_equal(d["divb"], divb_of(V, g))
+    if ic_type.startswith("OrszagTang") and "IC_noiseSeed" not in extra:
+        # B_x depends on y only and B_y on x only: the central differences
+        # cancel exactly (a transposed stencil would not)
+        assert not d["divb"].any()
+
+
+def divb_of(V, g):
+    """src/output.f90:262-273 on V (my, mx, 10) with guard cells, in its
+    order of operations: divb = 0 + 1/(2 dx) (Bx(i+1) - Bx(i-1)) + 1/(2 dy)
+    (By(j+1) - By(j-1))."""
+    s = g.ngc
+    js, is_ = slice(s, s + g.ny), slice(s, s + g.nx)
+    dx, dy = (g.xend - g.xbeg) / g.nx, (g.yend - g.ybeg) / g.ny
+    divb = np.zeros((g.ny, g.nx))
+    divb = divb + 1 / (2 * dx) * (V[js, s + 1:s + 1 + g.nx, 5] - V[js, s - 1:s - 1 + g.nx, 5])
+    divb = divb + 1 / (2 * dy) * (V[s + 1:s + 1 + g.ny, is_, 6] - V[s - 1:s - 1 + g.ny, is_, 6])
+    return divb
+
+
+def test_divb_closed_form():
+    """divb_of on a field with a known discrete divergence: B = (sin kx, cos
+    ly) gives sin(k dx) cos(kx) / dx - sin(l dy) sin(ly) / dy at each cell
+    centre (the central difference of a sinusoid, exactly)."""
+    g = ics.Grid(32, 24, 0.0, 1.0, 0.0, 2.0, R=2)
+    k, l = 2 * math.pi * 3, 2 * math.pi * 0.5
+    j, i = np.meshgrid(np.arange(g.my), np.arange(g.mx), indexing="ij")
+    dx, dy = 1.0 / 32, 2.0 / 24
+    x = (i - g.ngc + 0.5) * dx
+    y = (j - g.ngc + 0.5) * dy
+    V = np.zeros((g.my, g.mx, 10))
+    V[..., 5] = np.sin(k * x)
+    V[..., 6] = np.cos(l * y)
+    js, is_ = g.interior()
+    want = math.sin(k * dx) * np.cos(k * x[js, is_]) / dx - math.sin(l * dy) * np.sin(l * y[js, is_]) / dy
+    assert np.allclose(divb_of(V, g), want, rtol=0, atol=1e-12)
+
+
+# the reference's own parameter files (par/*_default.par), kept as data
+PAR_DIR = ROOT / "tests" / "golden" / "par"
+REF_PARS = sorted(PAR_DIR.glob("*_default.par"))
+
+
+def read_par(path):
+    """key -> value text, the first occurrence winning and '#' starting a
+    comment (src/readParamFile.f90)."""
+    out = {}
+    for ln in pathlib.Path(path).read_text().splitlines():
+        ln = ln.split("#", 1)[0].strip()
+        if "=" in ln:
+            k, v = (x.strip() for x in ln.split("=", 1))
+            out.setdefault(k, v)
+    return out
+
+
+def par_copy(tmp_path, src, name, **kw):
+    """src with sim_dataFileBaseName pointing into tmp_path and the keys in
+    kw replaced, every other line as it is."""
+    kw["sim_dataFileBaseName"] = f"'{tmp_path / name}'"
+    lines = []
+    for ln in pathlib.Path(src).read_text().splitlines():
+        key = ln.split("=", 1)[0].strip() if "=" in ln.split("#", 1)[0] else None
+        if key in kw:
+            ln = f"{key} = {kw.pop(key)}"
+        lines.append(ln)
+    assert not {k for k in kw if k != "sim_dataFileBaseName"}, kw
+    path = tmp_path / f"{name}.par"
+    path.write_text("\n".join(lines) + "\n")
+    return path
+
+
+def grid_of(p):
+    return ics.Grid(int(p["grid_Nx"]), int(p["grid_Ny"]), float(p["grid_xBeg"]), float(p["grid_xEnd"]),
+                    float(p["grid_yBeg"]), float(p["grid_yEnd"]), R=int(p["GP_maxRadius"]))
+
+
+@pytest.mark.skipif(not O.ref_available("exact"), reason="oracle/_ref is not built (make -C oracle ref)")
+@pytest.mark.parametrize("src", REF_PARS, ids=[p.stem for p in REF_PARS])
+def test_reference_par_initial_state_matches_initialCondition_set(tmp_path, src):
+    """The reference's default .par files through hmhd_driver --ic-only and
+    through the reference's own initialCondition_set
+    (src/initialCondition.f90:23-84, built from its sources into oracle/_ref):
+    output 0 holds the same primitives and conserved state bit for bit, and
+    its derived fields are those of the reference's state with its guard
+    cells (src/output.f90:248-274)."""
+    par = par_copy(tmp_path, src, "ic")
+    r = run_driver(par, "--ic-only")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    d = outfile.read_output(tmp_path / "ic_00000.hmhd")
+    ref = O.RefHarness(None, "exact", par_path=par)
+    U, V = ref.initial_condition()
+    ref.flush()
+    g = grid_of(read_par(par))
+    assert (ref.mx, ref.my, ref.ngc) == (g.mx, g.my, g.ngc)
+    js, is_ = g.interior()
+    for k, name in enumerate(PRIMS):
+        assert np.array_equal(d[name], V[js, is_, k]), name
+    assert np.array_equal(d["cons"], U[js, is_])
+    assert np.array_equal(d["divb"], divb_of(V, g))
+    magp = (V[js, is_, 5] ** 2 + V[js, is_, 6] ** 2 + V[js, is_, 7] ** 2) / 2
+    ener = V[js, is_, 0] * (V[js, is_, 1] ** 2 + V[js, is_, 2] ** 2 + V[js, is_, 3] ** 2) / 2 \
+        + V[js, is_, 0] * V[js, is_, 8]
+    assert np.array_equal(d["magp"], magp)
+    assert np.array_equal(d["ener"], ener + magp)
+    assert np.all(d["ordr"] == 2 * d["scheme"] + 1)
 
 
 def _lowbias32(x):
@@ -215,3 +320,51 @@ def test_driver_run_is_bit_identical_to_the_oracle_loop(tmp_path, name, extra, b
         for k, name in enumerate(PRIMS):
             assert np.array_equal(d[name], Vo[js, is_, k]), f"{name} at step {step}"
         assert np.array_equal(d["scheme"], orc.stage_schemes()[2][js, is_])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("src", REF_PARS, ids=[p.stem for p in REF_PARS])
+def test_reference_par_run_matches_the_oracle(tmp_path, src):
+    """Each reference default .par file (128^2, GP radius 3, l = 6 dx; the
+    Orszag-Tang vortex, the hydro explosion with CFL 0.5 and the hydro Sedov
+    blast) run by hmhd_driver with sim_nstepmax = 5: the final output equals
+    five steps of the oracle's driver loop from the driver's own output 0,
+    bit for bit, derived fields included."""
+    par = par_copy(tmp_path, src, "run", sim_nstepmax="5")
+    p = read_par(par)
+    print(f"driver on {src.name}", flush=True)
+    r = run_driver(par, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    files = sorted(tmp_path.glob("run_*.hmhd"))
+    d0, dN = outfile.read_output(files[0]), outfile.read_output(files[-1])
+    assert d0["step_number"] == 0 and dN["step_number"] == 5
+    g = grid_of(p)
+    bc = ics.BC_PERIODIC if p["sim_BC"].strip("'\"") == "periodic" else ics.BC_OUTFLOW
+    U = np.zeros((g.my, g.mx, 8))
+    V = np.zeros((g.my, g.mx, 10))
+    js, is_ = g.interior()
+    U[js, is_] = d0["cons"]
+    for k, name in enumerate(PRIMS):
+        V[js, is_, k] = d0[name]
+    ics.bc_fill(U, g, bc)
+    ics.bc_fill(V, g, bc)
+    cfg = O.Config(g.nx, g.ny, g.xbeg, g.xend, g.ybeg, g.yend, R=g.R, ell_over_delta=float(p["GP_ellOverDelta"]),
+                   gamma=float(p["sim_gamma"]), small_p=float(p["sim_smallPressure"]),
+                   small_rho=float(p["sim_smallDensity"]), small_B=float(p["sim_smallB"]),
+                   force_hydro=p["sim_forceHydro"].lower() == ".true.",
+                   bc=O.BC_PERIODIC if bc == ics.BC_PERIODIC else O.BC_OUTFLOW, cfl=float(p["sim_cfl"]))
+    orc = O.Oracle(cfg)
+    orc.set_state(U, V)
+    t = 0.0
+    for _ in range(5):
+        dt = orc.cfl_dt()
+        orc.advance(dt)
+        t += dt
+    Uo, Vo = orc.get_state()
+    assert dN["time"] == t
+    assert np.array_equal(dN["cons"], Uo[js, is_])
+    for k, name in enumerate(PRIMS):
+        assert np.array_equal(dN[name], Vo[js, is_, k]), name
+    assert np.array_equal(dN["scheme"], orc.stage_schemes()[2][js, is_])
+    assert np.array_equal(dN["divb"], divb_of(Vo, g))
+    assert np.all(dN["ordr"] == 2 * dN["scheme"] + 1)
