@@ -487,7 +487,7 @@ int hmhd_face_tuning_read(hmhd_face_tune* out, int32_t max, int32_t* n) {
         o.form = t[k].variant;
         o.tile_x = hmhd::kFaceVariants[t[k].variant].tx;
         o.tile_y = hmhd::kFaceVariants[t[k].variant].ty;
-        o.waves = t[k].R >= 3 ? 1 : hmhd::kFaceVariants[t[k].variant].waves;
+        o.waves = hmhd::kFaceVariants[t[k].variant].waves;
         for (int v = 0; v < 8; v++) o.ms[v] = t[k].ms[v];
     }
     return HMHD_OK;

@@ -1585,17 +1585,12 @@ static void launch_ff(const LaunchSet& L, int v, const Item* items, int n, int s
     const dim3 g = g1(n);
 #define HMHD_FF(TX, TY, W) \
     hipLaunchKernelGGL((k_face_full<R, ONE, TX, TY, W>), g, dim3(TX * TY), 0, st, L.insts, L.inst1, items, stage)
-    if constexpr (R < 3) {
-        switch (v) {
-            case 0: HMHD_FF(16, 16, 2); return;
-            case 1: HMHD_FF(32, 8, 2); return;
-            case 2: HMHD_FF(32, 16, 2); return;
-            case 3: HMHD_FF(16, 16, 1); return;
-            default: HMHD_FF(32, 8, 1); return;
-        }
-    } else {
-        if (v == 1 || v == 4) HMHD_FF(32, 8, 1);
-        else HMHD_FF(16, 16, 1);
+    switch (v) {
+        case 0: HMHD_FF(16, 16, 2); return;
+        case 1: HMHD_FF(32, 8, 2); return;
+        case 2: HMHD_FF(32, 16, 2); return;
+        case 3: HMHD_FF(16, 16, 1); return;
+        default: HMHD_FF(32, 8, 1); return;
     }
 #undef HMHD_FF
 }

@@ -29,10 +29,10 @@ void launch_end_iter(const LaunchSet& L, int w, uint32_t stamp, int stage, int32
 void launch_face_iter(const LaunchSet& L, TileList tl, int grid, int R, int stage, hipStream_t st);
 
 // The first pass's face kernel in its tuned forms: a tile shape (tx x ty
-// cells, one thread each) and a wave target (waves per SIMD the register
-// allocation allows).  R = 3 is built at one wave only, as its stencils
-// spill at two: its forms 0 and 3 both run 16 x 16, 1 and 4 both 32 x 8,
-// and 2 is not offered.
+// cells, one thread each) and a wave target (the waves per SIMD the
+// register budget given to the compiler must allow: 256 VGPRs at two, 512
+// at one).  Every radius needs 166 VGPRs or fewer, so the target changes
+// the compiler's scheduling, not the occupancy, which LDS bounds.
 struct FaceVariant {
     int tx, ty, waves;
 };
@@ -40,7 +40,7 @@ constexpr int kNumFaceVariants = 5;
 constexpr FaceVariant kFaceVariants[kNumFaceVariants] = {{16, 16, 2}, {32, 8, 2}, {32, 16, 2}, {16, 16, 1}, {32, 8, 1}};
 constexpr int kNumFaceShapes = 3;  // 16 x 16, 32 x 8, 32 x 16
 constexpr int face_shape_of(int v) { return v == 0 || v == 3 ? 0 : (v == 1 || v == 4 ? 1 : 2); }
-constexpr bool face_variant_valid(int R, int v) { return R < 3 || v == 3 || v == 4; }
+constexpr bool face_variant_valid(int v) { return v >= 0 && v < kNumFaceVariants; }
 // form v for the solves of radius R: `n` blocks, one per face tile (items:
 // the list of every active solve's face tiles of the form's shape; unused
 // for a single solve, whose tiles follow from the block index)

@@ -275,13 +275,17 @@ struct Sten {
 };
 
 // GP point value at quadrature point q of face f of the cell whose stencil
-// the accessor A is centred on; radius RR (src/reconstruct.f90:109-124)
+// the accessor A is centred on; radius RR (src/reconstruct.f90:109-124).
+// The stencil loop is unrolled eight cells at a time: unrolled in full, the
+// compiler hoists every stencil load of both face sides, and the first-pass
+// face kernel then needs more than 256 VGPRs; by eight it needs 152 and
+// runs three waves per SIMD.
 template <int RR, class A>
 __device__ __forceinline__ void gp_point(const A& a, const double* __restrict__ w, double* out) {
     constexpr Sten<RR> st{};
 #pragma unroll
     for (int v = 0; v < NCONS; v++) out[v] = 0.0;
-#pragma unroll
+#pragma unroll 8
     for (int k = 0; k < st.n; k++) {
         const double wk = w[k];
 #pragma unroll

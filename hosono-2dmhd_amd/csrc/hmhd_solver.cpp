@@ -587,9 +587,9 @@ void Engine::launch_face_iter_groups(TileList tl, int grid, int stage) {
 // bits: each face's arithmetic does not depend on which thread does it),
 // and the fastest is kept.
 int Engine::face_variant(int R, int stage) {
-    const int dflt = R < 3 ? 0 : 3, mode = g_tune_mode;
+    const int dflt = 0, mode = g_tune_mode;
     if (mode == 0) return dflt;
-    if (mode >= 2) return face_variant_valid(R, mode - 2) ? mode - 2 : dflt;
+    if (mode >= 2) return face_variant_valid(mode - 2) ? mode - 2 : dflt;
     const int cls = face_size_class(rcells_[R]);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -604,7 +604,6 @@ int Engine::face_variant(int R, int stage) {
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return dflt;
     double best = 1e30;
     for (int v = 0; v < kNumFaceVariants; v++) {
-        if (!face_variant_valid(R, v)) continue;
         const int k = face_shape_of(v);
         double t = 1e30;
         for (int rep = 0; rep < 3; rep++) {

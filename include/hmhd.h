@@ -134,9 +134,9 @@ int hmhd_synchronize(hmhd_t* h);
 
 /* The first pass's face kernel (the dominant one) comes in tuned forms:
  * form v = 0..HMHD_FACE_FORMS-1 is a tile of 16 x 16, 32 x 8 or 32 x 16
- * cells at two waves per SIMD (v = 0, 1, 2) or 16 x 16, 32 x 8 at one
- * (v = 3, 4); GP radius 3 runs at one wave only.  Every form computes the
- * same bits.  Process-wide, for every handle, batch and server:
+ * cells compiled for two waves per SIMD (v = 0, 1, 2) or 16 x 16, 32 x 8
+ * compiled for one (v = 3, 4), for every GP radius.  Every form computes
+ * the same bits.  Process-wide, for every handle, batch and server:
  * mode 1 (the default) measures every form on the first first pass of each
  * (radius, launch size class) and keeps the fastest; mode 0 always runs
  * form 0; mode 2 + v always runs form v.  No counterpart in the reference

@@ -266,10 +266,11 @@ def test_gpu_reproduces_reference_fixture(name):
 def test_every_face_kernel_form_is_bitwise(form):
     """Each tuned form of the first pass's face kernel (tile shape and wave
     target, forced) reproduces the oracle bit for bit: R = 2 Orszag-Tang on
-    a grid that is not a multiple of any tile, outflow blast, and R = 3."""
+    a grid that is not a multiple of any tile, outflow blast, R = 3 and the
+    R = 1 hydro explosion."""
     try:
         rt.face_tuning(form)
-        for name, n, R in (("ot", 50, 2), ("blast", 40, 2), ("ot_ref", 36, 3)):
+        for name, n, R in (("ot", 50, 2), ("blast", 40, 2), ("ot_ref", 36, 3), ("explosion_ref", 40, 1)):
             cfg, g, orc, gpu = _pair(name, n=n, R=R)
             for k in range(3):
                 _step_both(orc, gpu)
