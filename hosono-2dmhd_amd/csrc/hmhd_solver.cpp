@@ -86,6 +86,19 @@ void face_tuning_clear() {
 
 int face_size_class(long long cells) { return cells < (1 << 16) ? 0 : cells < (1 << 19) ? 1 : cells < (1 << 21) ? 2 : 3; }
 
+// The sequence words kernels store in pinned memory for the host to poll
+// (CFL results, MOOD snapshots).  They are unique within the process and
+// never 0: pinned memory is recycled between engines, and a word an earlier
+// engine left there must never equal the one a later engine waits for.
+static uint32_t unique_seq() {
+    static std::atomic<uint32_t> g_seq{0};
+    uint32_t v;
+    do {
+        v = g_seq.fetch_add(1, std::memory_order_relaxed) + 1;
+    } while (v == 0);
+    return v;
+}
+
 // ------------------------------------------------------------------ //
 
 int Engine::hip_check(hipError_t e, const char* what) {
@@ -315,6 +328,7 @@ int Engine::grow(T** p, size_t& cap, size_t need, bool host, unsigned host_flags
     int rc = host ? hip_check(hipHostMalloc(reinterpret_cast<void**>(p), sizeof(T) * n, host_flags), "hipHostMalloc")
                   : hip_check(hipMallocAsync(reinterpret_cast<void**>(p), sizeof(T) * n, st_), "hipMallocAsync");
     if (rc) return rc;
+    if (host) std::memset(*p, 0, sizeof(T) * n);  // no word of an earlier owner survives
     cap = n;
     return HMHD_OK;
 }
@@ -403,10 +417,12 @@ int Engine::commit() {
     if ((rc = put(pad, &d_pad_, cap_pad_))) return rc;
     // first-pass face tiles of each shape; padded cells per radius
     for (int R = 0; R <= kMaxR; R++) rcells_[R] = 0;
+    // the host vectors of the asynchronous copies live until the sync below
+    std::vector<Item> fl[kNumFaceShapes];
     for (int k = 0; k < kNumFaceShapes; k++) {
         const int v = k == 0 ? 0 : (k == 1 ? 1 : 2);
         const int tx = kFaceVariants[v].tx, ty = kFaceVariants[v].ty;
-        std::vector<Item> f;
+        std::vector<Item>& f = fl[k];
         for (size_t q = 0; q < n; q++) {
             const InstHost& h = inst_[q];
             if (!h.active) continue;
@@ -536,9 +552,8 @@ void Engine::prof_collect(bool wait_all) {
 
 // the sequence number the next snapshot into slot w carries (never 0)
 uint32_t Engine::next_seq(int w) {
-    if (++snap_seq_ == 0) snap_seq_ = 1;
-    snap_want_[w] = snap_seq_;
-    return snap_seq_;
+    snap_want_[w] = unique_seq();
+    return snap_want_[w];
 }
 
 // wait for snapshot w; its control words become ctl_, and its count slot
@@ -835,7 +850,7 @@ int Engine::cfl(std::vector<double>& dt_out) {
     int rc = commit();
     if (rc) return rc;
     const int n = (int)inst_.size();
-    if (++cfl_seq_ == 0) cfl_seq_ = 1;
+    cfl_seq_ = unique_seq();
     volatile int32_t* seq = reinterpret_cast<volatile int32_t*>(h_cfl_ + 2 * n);
     prof_begin(PF_CFL);
     launch_cfl(L, n, d_hcfl_, reinterpret_cast<uint32_t*>(d_ctl_ + (size_t)CTL_N * n + 5),

@@ -140,7 +140,7 @@ class Engine {
     // snapshot slot layout: CTL_N words per solve, two list counts, the sequence number
     size_t snap_stride() const { return (size_t)CTL_N * inst_.size() + 4; }
     uint32_t next_seq(int w);
-    uint32_t snap_seq_ = 0, snap_want_[2] = {0, 0};
+    uint32_t snap_want_[2] = {0, 0};
     // CFL maxima, two per solve, then a sequence word: pinned, written by the device
     unsigned long long* h_cfl_ = nullptr;
     unsigned long long* d_hcfl_ = nullptr;  // its device address

@@ -289,3 +289,26 @@ def test_face_tuning_table_records_every_form():
     e = [x for x in table if x["R"] == 2]
     assert e and len(e[0]["ms_per_form"]) == len(rt.FACE_FORMS)
     gpu.close()
+
+
+def test_fresh_handles_read_their_own_results():
+    """Handles created and destroyed one after another, each with one CFL
+    and one step on a state of its own: every dt and state is the oracle's.
+    The host polls sequence words in pinned memory that the runtime
+    recycles between handles; a word an earlier handle left there must not
+    pass for the awaited one (it did when each handle counted from 1)."""
+    for k in range(8):
+        name, n = (("ot", 96), ("blast", 40), ("sedov", 32), ("ot", 64))[k % 4]
+        cfg, g, U, V = cases.make(name, n=n)
+        orc = O.Oracle(cfg)
+        orc.set_state(U, V)
+        with rt.Solver(cases.runtime_config(cfg)) as gpu:
+            gpu.upload(U, V)
+            dt = gpu.cfl_computedt()
+            dt_ref = orc.cfl_dt()
+            assert dt == dt_ref, f"handle {k} ({name} {n}^2): dt {dt!r}, oracle {dt_ref!r}"
+            gpu.advance_solution(dt)
+            orc.advance(dt_ref)
+            Ug, _, _ = gpu.download(want_V=False, want_scheme=False)
+            Uo, _ = orc.get_state()
+            assert np.array_equal(Ug, Uo), f"handle {k} ({name} {n}^2): state after one step"
