@@ -62,8 +62,9 @@ PEAK_HBM_GBS = 8000.0
 # (+ - * / sqrt = 1; min/max/abs/compare = 0), run over the solve's states.
 FLOP_COUNT_EXE = PKG_DIR / "hmhd_flop_count"
 # compulsory HBM bytes of the face kernel per interior cell: read U (8 FP64),
-# write the raw x and y fluxes (16 FP64) and the two FLICD components (2 FP64)
-BYTES_FACE_KERNEL = 8 * 8 + 16 * 8 + 2 * 8
+# write the raw x and y fluxes (16 FP64), the four inadmissible-side flags
+# and the two last-flux radii (one byte each)
+BYTES_FACE_KERNEL = 8 * 8 + 16 * 8 + 6
 
 
 def count_flops(cfg, states, windows: int = 64) -> dict | None:
@@ -527,7 +528,7 @@ def plain_form(rt, args, cpu_proc=None):
     roofline = {
         "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-        "kernel": "k_face<2,FULL>", "flops_per_launch": flops_launch, "avg_launch_ms": avg_s * 1e3,
+        "kernel": "k_face_full<2> (the form in face_tuning)", "flops_per_launch": flops_launch, "avg_launch_ms": avg_s * 1e3,
         "launches": n_full,
         "peak_note": "FP64 vector (VALU) peak; the kernel uses no MFMA (the FP64 matrix peak is the same on MI355X)",
         "hbm_achieved_gbs": zones * BYTES_FACE_KERNEL / avg_s / 1e9, "hbm_peak_gbs": PEAK_HBM_GBS,

@@ -318,7 +318,10 @@ __device__ __forceinline__ void report_bad_cells(const Inst& I, uint32_t bits, i
 // thread (its x face at i-1/2 and its y face at j-1/2): every state of
 // [s-2, e+2]^2 at radius R, checked, and every flux of [s-1, e+2]^2 (the
 // range src/reconstruct.f90:81-83 and src/getFluxes.f90:26-30 cover).  The
-// per-tile bad_bits words were cleared by the stage set-up.
+// per-tile bad_bits words were cleared by the stage set-up.  FLB is not
+// written here: every cell is in rcF until the stage's first update, which
+// writes FLB at every cell of [s, e+1]^2 before anything reads it (the
+// update reads a neighbour's FLB only where that neighbour is not in rcF).
 template <int R, int TX, int TY>
 __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, int stage, double* __restrict__ sU,
                                                double* __restrict__ sAcc) {
@@ -394,7 +397,6 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
             if (fx) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v * NT];
-                I.FLB[idx] = acc[MAGY * NT];
                 I.fluxRR[idx] = (uint8_t)R;
             }
         }
@@ -425,7 +427,6 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
             if (fy) {
 #pragma unroll
                 for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v * NT];
-                I.FLB[fplane + idx] = acc[MAGX * NT];
                 I.fluxRR[fplane + idx] = (uint8_t)R;
             }
         }
