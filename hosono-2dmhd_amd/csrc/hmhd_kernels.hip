@@ -326,6 +326,11 @@ template <int R, int TX, int TY>
 __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, int stage, double* __restrict__ sU,
                                                double* __restrict__ sAcc) {
     constexpr int H = R + 1, NT = TX * TY;
+    // The flux sums over the Gauss points: in registers, except at radius 1,
+    // where the 16 registers they take would cost a wave per SIMD (194
+    // VGPRs); there they are a column of LDS per thread (stride NT).
+    constexpr bool kAccLds = R == 1;
+    constexpr int AS = kAccLds ? NT : 1;
     constexpr int LX = TX + 2 * R + 1, LP = LX * (TY + 2 * R + 1);
 
     if (I.R != R) return;
@@ -367,13 +372,11 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
         const LdsAcc<LX, LP> aC{sU + cc}, aW{sU + cc - 1}, aS{sU + cc - LX};
         uint8_t* const bad = I.inadm;  // planes [face][my][mx]
         constexpr int NQ = gp_nq(R);
-        // The flux sums over the Gauss points are kept in LDS (sAcc, a
-        // column per thread, stride NT), not in registers: the kernel is at
-        // its register limit, and the 16 registers spare it spills.
         // x-face: east state of (i-1, j) | west state of (i, j)
         if (do_x) {
             bool badL = false, badR = false;
-            double* const acc = sAcc + threadIdx.x;
+            double accr[NCONS];
+            double* const acc = kAccLds ? sAcc + threadIdx.x : accr;
 #pragma unroll 1
             for (int q = 0; q < NQ; q++) {
                 double uL[NCONS], uR[NCONS];
@@ -387,7 +390,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
                     hllc<0>(ph, uL, uR, F);
                     const double wq = c_qw[set][R][q];
 #pragma unroll
-                    for (int v = 0; v < NCONS; v++) acc[v * NT] = (q == 0 ? 0.0 : acc[v * NT]) + wq * F[v];
+                    for (int v = 0; v < NCONS; v++) acc[v * AS] = (q == 0 ? 0.0 : acc[v * AS]) + wq * F[v];
                 }
             }
             if (recW) bad[FACE_E * fplane + idx - 1] = badL;
@@ -396,14 +399,15 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
             if (recC && badR) bits |= bad_bits(rx, ry);
             if (fx) {
 #pragma unroll
-                for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v * NT];
+                for (int v = 0; v < NCONS; v++) I.FR[v * fplane + idx] = acc[v * AS];
                 I.fluxRR[idx] = (uint8_t)R;
             }
         }
         // y-face: north state of (i, j-1) | south state of (i, j)
         if (do_y) {
             bool badL = false, badR = false;
-            double* const acc = sAcc + threadIdx.x;
+            double accr[NCONS];
+            double* const acc = kAccLds ? sAcc + threadIdx.x : accr;
 #pragma unroll 1
             for (int q = 0; q < NQ; q++) {
                 double uL[NCONS], uR[NCONS];
@@ -417,7 +421,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
                     hllc<1>(ph, uL, uR, F);
                     const double wq = c_qw[set][R][q];
 #pragma unroll
-                    for (int v = 0; v < NCONS; v++) acc[v * NT] = (q == 0 ? 0.0 : acc[v * NT]) + wq * F[v];
+                    for (int v = 0; v < NCONS; v++) acc[v * AS] = (q == 0 ? 0.0 : acc[v * AS]) + wq * F[v];
                 }
             }
             if (recS) bad[FACE_N * fplane + idx - mx] = badL;
@@ -426,7 +430,7 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
             if (recC && badR) bits |= bad_bits(rx, ry);
             if (fy) {
 #pragma unroll
-                for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v * NT];
+                for (int v = 0; v < NCONS; v++) I.FR[(NCONS + v) * fplane + idx] = acc[v * AS];
                 I.fluxRR[fplane + idx] = (uint8_t)R;
             }
         }
@@ -688,7 +692,7 @@ template <int R, bool ONE, int TX, int TY, int WAVES>
 __global__ __launch_bounds__(TX * TY, WAVES) void k_face_full(const Inst* __restrict__ insts, const Inst one,
                                                              const Item* __restrict__ items, int stage) {
     __shared__ double sU[NCONS * (TX + 2 * R + 1) * (TY + 2 * R + 1)];
-    __shared__ double sAcc[NCONS * TX * TY];
+    __shared__ double sAcc[R == 1 ? NCONS * TX * TY : 1];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     Item it;
     if (ONE) {

@@ -278,8 +278,8 @@ struct Sten {
 // the accessor A is centred on; radius RR (src/reconstruct.f90:109-124).
 // The stencil loop is unrolled eight cells at a time: unrolled in full, the
 // compiler hoists every stencil load of both face sides, and the first-pass
-// face kernel then needs more than 256 VGPRs; by eight it needs 152 and
-// runs three waves per SIMD.
+// face kernel then needs more than 256 VGPRs; by eight it needs at most 166
+// and runs three waves per SIMD.
 template <int RR, class A>
 __device__ __forceinline__ void gp_point(const A& a, const double* __restrict__ w, double* out) {
     constexpr Sten<RR> st{};
