@@ -1022,6 +1022,76 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = rk_combine(stage, ub[v], un[v]);
 }
 
+// The stage's first update in a ready first pass: every cell is in rc and
+// rcF (they are set everywhere until the first update) and its scheme is R.
+// The arithmetic is update_tile's; the loads come in two phases -- the
+// FLICD stencil, the fluxes and U for the FV update and the RK combination,
+// then U around the cell for detection -- so that the kernel needs fewer
+// registers and more waves hide the loads of a full grid.
+__device__ __forceinline__ void update_tile_full(const Inst& I, const Ctl& cw, const Item it, int stage) {
+    if (!pass_ready(cw)) return;
+    const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    if (i < s || j < s || i > ex + 1 || j > ey + 1) return;
+    const size_t idx = (size_t)j * mx + i, fp = (size_t)mx * I.my;
+    const bool inner = i <= ex && j <= ey;
+    const double* __restrict__ X = I.FR + MAGY * fp;
+    const double* __restrict__ Y = I.FR + (NCONS + MAGX) * fp;
+    const double xw = X[idx - 1], x0 = X[idx], x1 = X[idx + 1], x2 = X[idx + 2];
+    const double xs0 = X[idx - mx], xs1 = X[idx - mx + 1], xn0 = X[idx + mx], xn1 = X[idx + mx + 1];
+    const double ys = Y[idx - mx], y0 = Y[idx], y1 = Y[idx + mx], y2 = Y[idx + 2 * mx];
+    const double yw0 = Y[idx - 1], yw1 = Y[idx - 1 + mx], ye0 = Y[idx + 1], ye1 = Y[idx + 1 + mx];
+    const StageBufs sb = stage_bufs(I.rot, stage);
+    const double* __restrict__ Uin = ubuf(I, sb.in);
+    const size_t plane = (size_t)I.nx * I.ny;
+    const size_t ci = inner ? (size_t)(j - s) * I.nx + (i - s) : 0;
+    double fxm[NCONS], fxp[NCONS], fym[NCONS], fyp[NCONS], uc[NCONS], ub[NCONS];
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) {
+        if (v == MAGX || v == MAGY) continue;
+        fxm[v] = I.FR[v * fp + idx];
+        fxp[v] = I.FR[v * fp + idx + 1];
+        fym[v] = I.FR[(NCONS + v) * fp + idx];
+        fyp[v] = I.FR[(NCONS + v) * fp + idx + mx];
+    }
+    const double* __restrict__ Ub = ubuf(I, sb.base);
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) {
+        uc[v] = Uin[v * plane + ci];
+        ub[v] = Ub[v * plane + ci];
+    }
+    const double cx = flicd_corr(x0, x1, xw, y1, y0, yw0, yw1);
+    const double cy = flicd_corr(y0, y1, ys, x1, x0, xs0, xs1);
+    I.FLB[idx] = cx;
+    I.FLB[fp + idx] = cy;
+    if (!inner) return;
+    // the east and north neighbours are in rcF: their corrections
+    const double cxe = flicd_corr(x1, x2, x0, ye1, ye0, y0, y1);
+    const double cyn = flicd_corr(y1, y2, y0, xn1, xn0, x0, x1);
+    fxm[MAGX] = 0.0;
+    fxp[MAGX] = 0.0;
+    fym[MAGX] = cy;
+    fyp[MAGX] = cyn;
+    fxm[MAGY] = cx;
+    fxp[MAGY] = cxe;
+    fym[MAGY] = 0.0;
+    fyp[MAGY] = 0.0;
+    const double dtdx = I.dtdx, dtdy = I.dtdy;
+    double un[NCONS];
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) un[v] = fv_update(uc[v], dtdx, fxp[v], fxm[v], dtdy, fyp[v], fym[v]);
+    double* __restrict__ Uout = ubuf(I, sb.out);
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = rk_combine(stage, ub[v], un[v]);
+    reload_barrier();  // detection's loads after the first phase's
+    Around a;
+    load_around(I, Uin, i, j, a);
+    const Phys ph = phys_of(I);
+    const bool lower = I.scheme[idx] > 0 && detect_cell(I, ph, a, un, uc);
+    I.low[idx] = lower ? 1 : 0;
+    if (lower) I.ctl[CTL_ANY_LOWER] = 1;
+}
+
 template <bool ONE>
 __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
                                                 int stage, uint32_t* __restrict__ marks,
@@ -1075,6 +1145,43 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
         }
         update_tile(I, cw, it, stage);
     }, c1.run | c1.any_inadm | c1.mode, stamp != 0 && nomarks);
+}
+
+// The stage's first update (iteration 0, every tile of every solve): the
+// control words of k_update's block 0, the scheme reset to R, and, in a
+// ready pass, update_tile_full.  At most 168 VGPRs: three waves per SIMD.
+template <bool ONE>
+__global__ __launch_bounds__(256, 3) void k_update_first(const Inst* __restrict__ insts, const Inst one,
+                                                         const TileList tl, int stage, int ninst) {
+    const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < ninst; k += blockDim.x) {
+            const Inst& I = inst_at<ONE>(insts, one, k);
+            const Ctl c = ONE ? c1 : ctl_of(I.ctl);
+            I.ctl[CTL_READY] = pass_ready(c) ? 1 : 0;
+            I.ctl[CTL_SYNC] = sync_due(I, c) ? 1 : 0;
+            I.ctl[CTL_LOWER] = c.run != 0 && c.any_inadm != 0 ? 1 : 0;
+        }
+    }
+    if (ONE && !pass_ready(c1)) {
+        // a single solve whose pass is not ready: only the scheme's reset,
+        // by tile index (its tiles are its rows of tiles in order)
+        const int nt = one.ntx * ((one.my + kCellT - 1) / kCellT);
+        const int lx = threadIdx.x % kCellT, ly = threadIdx.x / kCellT;
+        for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+            const int i = (t % one.ntx) * kCellT + lx, j = (t / one.ntx) * kCellT + ly;
+            if (i < one.mx && j < one.my) one.scheme[(size_t)j * one.mx + i] = (uint8_t)one.R;
+        }
+        return;
+    }
+    for_tiles(tl, [&](const Item it) {
+        const Inst& I = inst_at<ONE>(insts, one, it.inst);
+        const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
+        // the stage set-up's scheme = R (src/timeStep.f90:79), read first here
+        const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+        if (i < I.mx && j < I.my) I.scheme[(size_t)j * I.mx + i] = (uint8_t)I.R;
+        update_tile_full(I, cw, it, stage);
+    }, c1.run | c1.any_inadm | c1.mode, false);
 }
 
 // End of a MOOD iteration, one block.  Each instance still in its loop
@@ -1616,6 +1723,10 @@ void launch_face_full(const LaunchSet& L, int R, int v, const Item* items, int n
 
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st) {
     // block 0 always runs: it writes the control words the iteration's end reads
+    if (stamp == 0) {
+        HMHD_LAUNCH1(k_update_first, g1(grid > 0 ? grid : 1), dim3(256), tl, stage, L.ninst);
+        return;
+    }
     HMHD_LAUNCH1(k_update, g1(grid > 0 ? grid : 1), dim3(256), tl, stage, L.marks, L.amarks, stamp, L.ninst);
 }
 #undef HMHD_LAUNCH1
