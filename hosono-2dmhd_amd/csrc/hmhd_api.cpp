@@ -105,9 +105,10 @@ void set_free_error(const char* msg) { g_create_err = msg; }
 // holds one large solve at a time, which then steps alone with the
 // single-solve kernels instead of waiting in lockstep for the batch.  A
 // request of at least kSoloCells interior cells takes a free solo lane if
-// there is one, otherwise it joins the batch.
+// there is one, otherwise it joins the batch.  Four lanes in all, one per
+// hardware queue a process gets by default.
 struct hmhd_server_handle {
-    static constexpr int kSoloLanes = 2;
+    static constexpr int kSoloLanes = 3;
     static constexpr long long kSoloCells = 512 * 512;
 
     struct Req {

@@ -213,7 +213,7 @@ int hmhd_batch_profile_read(hmhd_batch_t* b, double* ms, int64_t* launches, int3
  * stepped with the other admitted runs, and released when they finish.
  * Admitted runs go to lanes that overlap on the device, each an engine
  * with its own stream and worker thread: one lane steps any number of runs
- * together, two more hold one large run (512^2 cells or more) each.  The
+ * together, three more hold one large run (512^2 cells or more) each.  The
  * calls below are thread-safe.  Results are bit-identical to the same run
  * on its own handle.  A run that finds all GP table sets in use waits for
  * a running one to finish.  The reference has no served form.
