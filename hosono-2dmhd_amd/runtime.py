@@ -111,7 +111,9 @@ _LIB = None
 
 
 def load_library(path: pathlib.Path | str | None = None):
-    """Load libhmhd.so; raises if it has not been built."""
+    """Load libhmhd.so; raises if it has not been built.  With `path`, the
+    library at that path is loaded and becomes the one every later Solver,
+    Batch and Server of this process uses."""
     global _LIB
     if _LIB is not None and path is None:
         return _LIB
@@ -167,8 +169,7 @@ def load_library(path: pathlib.Path | str | None = None):
     lib.hmhd_server_poll.argtypes = [vp, i64, C.POINTER(_Result)]
     lib.hmhd_server_wait.argtypes = [vp, i64, C.c_double, C.POINTER(_Result), dp, dp, ip]
     lib.hmhd_server_stats_read.argtypes = [vp, C.POINTER(_ServerStats)]
-    if path is None:
-        _LIB = lib
+    _LIB = lib
     return lib
 
 
