@@ -403,6 +403,21 @@ def served_form(rt, args) -> dict:
 
 # ---------------------------------------------------------------------- #
 
+def face_form_used(rt, n: int, forced: str = "tune") -> str | None:
+    """The first-pass face kernel's form ("TXxTY/wW") for radius 2 at an
+    n x n grid's launch size class: the forced one, or the one this
+    process's tuning table holds."""
+    if forced != "tune":
+        tx, ty, w = rt.FACE_FORMS[int(forced)]
+        return f"{tx}x{ty}/w{w}"
+    padded = (n + 8) * (n + 8)
+    cls = 0 if padded < (1 << 16) else 1 if padded < (1 << 19) else 2 if padded < (1 << 21) else 3
+    for e in rt.face_tuning_table():
+        if e["R"] == 2 and e["size_class"] == cls:
+            return f"{e['tile'][0]}x{e['tile'][1]}/w{e['waves']}"
+    return None
+
+
 def plain_form(rt, args, cpu_proc=None):
     """The metric's workload: one Orszag-Tang solve from t = 0 to t_end.
 
@@ -521,14 +536,17 @@ def plain_form(rt, args, cpu_proc=None):
         raise RuntimeError(f"hmhd_flop_count failed or is not built: {flops}")
     flops_launch = flops["face_flops_per_launch"]
     achieved = flops_launch / avg_s / 1e12
+    # the form the face kernel ran in, and its HBM bytes per launch as two
+    # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) measured it for that form
+    form = face_form_used(rt, args.n, args.face_form)
     traffic = None
     tf = ROOT / "profiles" / "face_kernel_traffic_ot1024.json"
-    if tf.exists() and args.n == 1024:
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    if tf.exists() and args.n == 1024 and form is not None:
+        traffic = json.loads(tf.read_text()).get("forms", {}).get(form, {}).get("bytes_per_launch")
     roofline = {
         "bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-        "kernel": "k_face_full<2> (the form in face_tuning)", "flops_per_launch": flops_launch, "avg_launch_ms": avg_s * 1e3,
+        "kernel": f"k_face_full<2> in form {form}", "flops_per_launch": flops_launch, "avg_launch_ms": avg_s * 1e3,
         "launches": n_full,
         "peak_note": "FP64 vector (VALU) peak; the kernel uses no MFMA (the FP64 matrix peak is the same on MI355X)",
         "hbm_achieved_gbs": zones * BYTES_FACE_KERNEL / avg_s / 1e9, "hbm_peak_gbs": PEAK_HBM_GBS,
@@ -579,6 +597,8 @@ def main() -> int:
     ap.add_argument("--blast-n", type=int, default=2048)
     ap.add_argument("--blast-tmax", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--face-form", default="tune",
+                    help="first-pass face kernel form: tune (default) or an index into runtime.FACE_FORMS")
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--served-requests", type=int, default=48)
@@ -603,6 +623,8 @@ def main() -> int:
 
     load_package()
     import hosono_2dmhd_amd.runtime as rt
+    if args.face_form != "tune":
+        rt.face_tuning(int(args.face_form))
     line = plain_form(rt, args, cpu)
     print(f"plain: {line['value']:.4g} zone-updates/s; full solve {line['full_solve']['value']:.4g}",
           file=sys.stderr, flush=True)

@@ -1054,11 +1054,12 @@ __device__ __forceinline__ void update_tile_full(const Inst& I, const Ctl& cw, c
         fym[v] = I.FR[(NCONS + v) * fp + idx];
         fyp[v] = I.FR[(NCONS + v) * fp + idx + mx];
     }
+    // the RK base state: stage 0 does not combine (U1 = Unew), so it reads none
     const double* __restrict__ Ub = ubuf(I, sb.base);
 #pragma unroll
     for (int v = 0; v < NCONS; v++) {
         uc[v] = Uin[v * plane + ci];
-        ub[v] = Ub[v * plane + ci];
+        ub[v] = stage != 0 ? Ub[v * plane + ci] : 0.0;
     }
     const double cx = flicd_corr(x0, x1, xw, y1, y0, yw0, yw1);
     const double cy = flicd_corr(y0, y1, ys, x1, x0, xs0, xs1);
