@@ -282,8 +282,15 @@ struct hmhd_server_handle {
                 continue;
             }
             // results of the finished solves, outside the lock: nobody reads
-            // a request's arrays before it is marked done
-            std::vector<std::pair<int64_t, int32_t>> done;
+            // a request's arrays before it is marked done (its `res`, which
+            // poll reads at any time, is written under the lock below)
+            struct Done {
+                int64_t rid;
+                int32_t state, steps;
+                int64_t zone_updates;
+                double t;
+            };
+            std::vector<Done> done;
             for (int64_t sid : fin) {
                 const int64_t rid = L.by_solve[sid];
                 Req* r;
@@ -300,10 +307,7 @@ struct hmhd_server_handle {
                 int32_t state = ri.state == hmhd::RUN_DONE ? HMHD_REQ_DONE : HMHD_REQ_ABORTED;
                 if (L.eng.download(sid, r->Uout.data(), r->Vout.data(), r->Sout.data()) != HMHD_OK)
                     state = HMHD_REQ_FAILED;
-                r->res.steps = ri.steps;
-                r->res.t = ri.t;
-                r->res.zone_updates = ri.zone_updates;
-                done.emplace_back(rid, state);
+                done.push_back(Done{rid, state, ri.steps, ri.zone_updates, ri.t});
                 (void)L.eng.remove(sid);
                 L.run.forget(sid);
                 L.by_solve.erase(sid);
@@ -311,10 +315,13 @@ struct hmhd_server_handle {
             std::lock_guard<std::mutex> lk(mu);
             busy_s += b - a;
             st.engine_steps += 1;
-            for (auto& [rid, state] : done) {
-                Req& r = reqs[rid];
-                if (state == HMHD_REQ_FAILED) err = L.eng.err;
-                finish_locked(r, state);
+            for (const Done& d : done) {
+                Req& r = reqs[d.rid];
+                if (d.state == HMHD_REQ_FAILED) err = L.eng.err;
+                r.res.steps = d.steps;
+                r.res.t = d.t;
+                r.res.zone_updates = d.zone_updates;
+                finish_locked(r, d.state);
                 release_locked(L, r);
                 st.completed += 1;
                 st.zone_updates += r.res.zone_updates;
