@@ -10,12 +10,14 @@ step with every MOOD pass (timeStep_advanceSolution).  A zone-update is one
 interior cell advanced one step.
 
 The solve runs from t = 0 to t = 0.5 and is reported whole as
-`full_solve`.  With --steps K the line's value is exactly K of its steps,
-spread evenly over the solve in simulated time (the step starting at the
-first t >= (k + 1/2) * 0.5 / K), each bracketed by stream synchronizations:
-the MOOD loop's depth varies over the solve (1 pass per stage early, about
-8 near t = 0.25, 20.5 on average), so a contiguous window would not stand
-for the configuration.  With no --steps the value is the whole solve.
+`full_solve`.  With --steps K the line's value is exactly K of its N
+steps, one from each K-th of the solve (by step index), the one whose MOOD
+pass count is nearest that stretch's mean, timed in a second solve from
+the same initial state, each bracketed by stream synchronizations: the
+MOOD loop's depth varies over the solve (1 pass per stage early, about 8
+near t = 0.2, 20.5 on average, with large step-to-step swings), so neither
+a contiguous window nor a handful of evenly spaced steps stands for the
+configuration.  With no --steps the value is the whole solve.
 Warm-up steps run on a separate copy before the timed region.
 
 One JSON line on stdout.  `roofline` covers the dominant kernel, the face
@@ -422,12 +424,15 @@ def plain_form(rt, args, cpu_proc=None):
     """The metric's workload: one Orszag-Tang solve from t = 0 to t_end.
 
     The whole solve is `full_solve`.  With --steps K the line's value is
-    exactly K of its steps, spread evenly over the solve: the step that
-    starts at the first t >= (k + 1/2) t_end / K, k = 0..K-1, each
-    bracketed by stream synchronizations, so the sample has the solve's mix
-    of quiet and MOOD-heavy steps; without --steps the value is the whole
-    solve.  The GPU states at t = (j + 1/2) t_end / 4, j = 0..3, go to the
-    CPU baseline's child (the downloads are outside every timed region)."""
+    exactly K of its N steps: a second solve from the same initial state
+    times, from each K-th of the solve by step index, the step whose MOOD
+    pass count is nearest that stretch's mean, each bracketed by stream
+    synchronizations, so the sample has the solve's mix of quiet and
+    MOOD-heavy steps (stretches of steps, not of simulated time: the
+    MOOD-heavy phases take more, shorter steps);
+    without --steps the value is the whole solve.  The GPU states at t =
+    (j + 1/2) t_end / 4, j = 0..3, go to the CPU baseline's child (the
+    downloads are outside every timed region)."""
     import numpy as np
     g, U, V = workload(args.n)
     cfg = rt.Config(args.n, args.n, R=2, ell_over_delta=12.0)
@@ -440,31 +445,27 @@ def plain_form(rt, args, cpu_proc=None):
         w.synchronize()
 
     # HIP events bracket only the first-pass face kernel, except in the
-    # breakdown window, where every kernel family is timed (but the timed
-    # sample's steps)
+    # breakdown window, where every kernel family is timed
     s = rt.Solver(cfg)
     s.upload(U, V)
     s.profile(["face_full"])
     s.synchronize()
     s.profile_read(reset=True)
     tmax, K, zones = args.tmax, args.steps, args.n * args.n
-    samp_t = [(k + 0.5) * tmax / K for k in range(K)] if K > 0 else []
     # states at t = (j + 1/2) t_end / 4: the CPU baseline's start states and
     # the operation count's inputs
     cpu_t = [(j + 0.5) * tmax / 4 for j in range(4)]
-    si = ci = 0
+    ci = 0
     state_paths = []
     samp_s, samp_n, samp_passes, samp_first = 0.0, 0, 0, []
     t, steps, passes = 0.0, 0, 0
+    step_passes = []    # MOOD passes of every step of the whole solve
     bd = None           # [first step, profile snapshot, wall start] of the breakdown window
     breakdown, bd_range, bd_wall = {}, None, None
-    bd_first = None if K > 0 else args.breakdown_at
     full_s = 0.0
     seg0 = time.perf_counter()
     while t < tmax:
-        sample = si < len(samp_t) and t >= samp_t[si]
-        save = ci < len(cpu_t) and t >= cpu_t[ci]
-        if sample or save:
+        if ci < len(cpu_t) and t >= cpu_t[ci]:
             s.synchronize()
             full_s += time.perf_counter() - seg0
             while ci < len(cpu_t) and t >= cpu_t[ci]:
@@ -477,7 +478,7 @@ def plain_form(rt, args, cpu_proc=None):
                 ci += 1
             s.synchronize()
             seg0 = time.perf_counter()
-        if bd is None and bd_first is not None and steps == bd_first and args.breakdown_steps > 0:
+        if bd is None and steps == args.breakdown_at and args.breakdown_steps > 0:
             s.profile(True)
             bd = [steps, s.profile_read(), time.perf_counter()]
         elif bd is not None and bd_range is None and steps == bd[0] + args.breakdown_steps:
@@ -486,31 +487,47 @@ def plain_form(rt, args, cpu_proc=None):
             s.profile(["face_full"])
             breakdown = {k: (snap1[k][0] - bd[1][k][0], snap1[k][1] - bd[1][k][1]) for k in snap1}
             bd_range = [bd[0], steps]
-        in_bd = bd is not None and bd_range is None
-        if sample and in_bd:
-            s.profile(["face_full"])
-        t_step = time.perf_counter()
         dt = rt.valid_time_step(s.cfl_computedt(), t, tmax)
         s.advance_solution(dt)
-        p = int(s.stats()[0].sum())
-        if sample:
+        t += dt
+        steps += 1
+        step_passes.append(int(s.stats()[0].sum()))
+        passes += step_passes[-1]
+    s.synchronize()
+    full_s += time.perf_counter() - seg0
+    prof = s.profile_read()
+    if K > 0:
+        # the timed sample: a second solve from the same initial state, one
+        # step from each K-th of the N steps the whole solve took, the one
+        # whose MOOD pass count is nearest its stretch's mean
+        want = []
+        for k in range(K):
+            a, b = k * steps // K, max((k + 1) * steps // K, k * steps // K + 1)
+            m = sum(step_passes[a:b]) / (b - a)
+            want.append(min(range(a, b), key=lambda i: (abs(step_passes[i] - m), i)))
+        want = sorted(set(want))  # fewer than K when the solve has fewer than K steps
+        s.upload(U, V)
+        s.profile([])
+        s.synchronize()
+        t2, n2 = 0.0, 0
+        for idx in want:
+            while n2 < idx:
+                dt = rt.valid_time_step(s.cfl_computedt(), t2, tmax)
+                s.advance_solution(dt)
+                t2 += dt
+                n2 += 1
+            s.synchronize()
+            t_step = time.perf_counter()
+            dt = rt.valid_time_step(s.cfl_computedt(), t2, tmax)
+            s.advance_solution(dt)
+            p = int(s.stats()[0].sum())
             s.synchronize()
             samp_s += time.perf_counter() - t_step
             samp_n += 1
             samp_passes += p
-            samp_first.append(steps)
-            if in_bd:
-                s.profile(True)
-            while si < len(samp_t) and t >= samp_t[si]:
-                si += 1
-            if si >= len(samp_t) // 2 + 1 and bd_first is None:
-                bd_first = steps + 1  # the breakdown window starts after the middle sample
-        t += dt
-        steps += 1
-        passes += p
-    s.synchronize()
-    full_s += time.perf_counter() - seg0
-    prof = s.profile_read()
+            samp_first.append(n2)
+            t2 += dt
+            n2 += 1
     s.close()
     cpu = finish_cpu_baseline(cpu_proc, [q for q, _ in state_paths]) if cpu_proc is not None else None
     flops = count_flops(cfg, [np.load(q)["U"] for q, _ in state_paths] or [U])
@@ -519,12 +536,13 @@ def plain_form(rt, args, cpu_proc=None):
 
     full = {"value": steps * zones / full_s, "unit": UNIT, "steps": steps, "seconds": full_s, "t_end": t,
             "ms_per_step": full_s / steps * 1e3, "mood_passes_per_stage": passes / (3 * steps),
-            "note": "the whole solve from t=0 in the same run; it contains the sampled steps, whose "
-                    "synchronizations cost it a little, and the breakdown window, whose events do"}
+            "note": "the whole solve from t=0 in the same run; it contains the breakdown window, whose "
+                    "events cost it a little"}
     if K > 0 and samp_n > 0:
         value, n_timed, sec = samp_n * zones / samp_s, samp_n, samp_s
         sampled = {"steps": samp_first, "passes_per_stage": samp_passes / (3 * samp_n),
-                   "rule": f"the step starting at the first t >= (k + 1/2) * {tmax} / {K}, k = 0..{K - 1}, "
+                   "rule": f"one step from each of {K} equal stretches of the {steps}-step solve, the one whose "
+                           f"MOOD pass count is nearest the stretch's mean, timed in a second solve from t = 0, "
                            f"each bracketed by stream synchronizations"}
     else:
         value, n_timed, sec = full["value"], steps, full_s
@@ -587,7 +605,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--tmax", type=float, default=0.5)
     ap.add_argument("--breakdown-at", type=int, default=700,
-                    help="first step of the all-kernel timing window when --steps is 0 (else after the middle sample)")
+                    help="first step of the all-kernel timing window in the whole solve")
     ap.add_argument("--breakdown-steps", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=20, help="steps in the CPU sample (split over its four states)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0, help="cap on the CPU sample's seconds")
