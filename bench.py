@@ -405,6 +405,24 @@ def served_form(rt, args) -> dict:
 
 # ---------------------------------------------------------------------- #
 
+def representative_steps(step_passes: list[int], K: int) -> list[int]:
+    """Indices of K steps of a solve whose steps took `step_passes` MOOD
+    passes: the solve cut by step index into K stretches as equal as
+    possible, and from each the first step whose pass count is nearest the
+    stretch's mean.  Sorted; fewer than K (no repeats) when the solve has
+    fewer than K steps."""
+    n = len(step_passes)
+    want = set()
+    for k in range(K):
+        a = k * n // K
+        b = max((k + 1) * n // K, a + 1)
+        if a >= n:
+            continue
+        m = sum(step_passes[a:b]) / (b - a)
+        want.add(min(range(a, b), key=lambda i: (abs(step_passes[i] - m), i)))
+    return sorted(want)
+
+
 def face_form_used(rt, n: int, forced: str = "tune") -> str | None:
     """The first-pass face kernel's form ("TXxTY/wW") for radius 2 at an
     n x n grid's launch size class: the forced one, or the one this
@@ -500,12 +518,7 @@ def plain_form(rt, args, cpu_proc=None):
         # the timed sample: a second solve from the same initial state, one
         # step from each K-th of the N steps the whole solve took, the one
         # whose MOOD pass count is nearest its stretch's mean
-        want = []
-        for k in range(K):
-            a, b = k * steps // K, max((k + 1) * steps // K, k * steps // K + 1)
-            m = sum(step_passes[a:b]) / (b - a)
-            want.append(min(range(a, b), key=lambda i: (abs(step_passes[i] - m), i)))
-        want = sorted(set(want))  # fewer than K when the solve has fewer than K steps
+        want = representative_steps(step_passes, K)
         s.upload(U, V)
         s.profile([])
         s.synchronize()
