@@ -598,9 +598,9 @@ void Engine::launch_face_iter_groups(TileList tl, int grid, int stage) {
 
 // The first-pass face kernel's form for the solves of radius R, from the
 // tuning table; on the first launch of a (radius, size class) pair every
-// form runs three times on this first pass (they all compute the same
-// bits: each face's arithmetic does not depend on which thread does it),
-// and the fastest is kept.
+// form runs once untimed and then three times timed on this first pass
+// (they all compute the same bits: each face's arithmetic does not depend
+// on which thread does it), and the fastest is kept.
 int Engine::face_variant(int R, int stage) {
     const int dflt = 0, mode = g_tune_mode;
     if (mode == 0) return dflt;
@@ -618,6 +618,11 @@ int Engine::face_variant(int R, int stage) {
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return dflt;
     double best = 1e30;
+    // one untimed launch of every form first: the first launches of a
+    // process run on cold caches and clocks, which would rank the forms
+    // by launch order rather than by speed
+    for (int v = 0; v < kNumFaceVariants; v++) launch_face_full(L, R, v, d_fitems_[face_shape_of(v)],
+                                                                 nfitems_[face_shape_of(v)], stage, st_);
     for (int v = 0; v < kNumFaceVariants; v++) {
         const int k = face_shape_of(v);
         double t = 1e30;
