@@ -438,19 +438,21 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
     report_bad_cells(I, bits, i, j);
 }
 
-// One face of a later MOOD iteration on the four lanes of a quad, lane q
-// for Gauss point q (src/reconstruct.f90:109-133, src/riemannSolver.f90:
-// 37-44): the lanes rebuild their point of each side to be rebuilt (at
-// the current radius) or needed for the flux (at the radius the side was
-// last rebuilt with), check its admissibility, and form w_q F_HLLC; lane 0
-// sums the points in ascending order from zero (or takes the one
-// first-order flux as it is) and writes the face's results.
+// One face of a later MOOD iteration on a group of R + 1 lanes (the Gauss
+// points of radius R), lanes base .. base + R of the wave, lane base + q
+// for point q (src/reconstruct.f90:109-133, src/riemannSolver.f90:37-44):
+// the lanes rebuild their point of each side to be rebuilt (at the current
+// radius) or needed for the flux (at the radius the side was last rebuilt
+// with), check its admissibility, and form w_q F_HLLC; lane q = 0 sums the
+// points in ascending order from zero (or takes the one first-order flux
+// as it is) and writes the face's results.
 // rr_now: the face's radius, min of its cells' schemes; storedL/R: the
 // radius each side was last reconstructed with.
 template <int R, int DIR, class A>
-__device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set, const A& aL, const A& aR, size_t idx,
+__device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, int set, const A& aL, const A& aR, size_t idx,
                                           size_t nb, bool recL, bool recR, bool flux, int rr_now, int storedL,
-                                          int storedR, int q, int lx, int ly, uint32_t& bits) {
+                                          int storedR, int q, int base, int lx, int ly, uint32_t& bits) {
+    constexpr int G = gp_nq(R);  // lanes of the group
     const size_t fplane = (size_t)I.mx * I.my;
     constexpr int fL = DIR == 0 ? FACE_E : FACE_N, fR = DIR == 0 ? FACE_W : FACE_S;
     const int rrL = recL ? rr_now : storedL;
@@ -474,22 +476,26 @@ __device__ __forceinline__ void face_quad(const Inst& I, const Phys& ph, int set
 #pragma unroll
         for (int v = 0; v < NCONS; v++) t[v] = 0.0;
     }
-    // every lane of the quad takes part in the exchanges
-    const int base = (int)(threadIdx.x & 63u) & ~3;
+    // every lane of the group takes part in the exchanges
     double acc[NCONS];
 #pragma unroll
     for (int v = 0; v < NCONS; v++) {
-        const double t1 = __shfl(t[v], base + 1), t2 = __shfl(t[v], base + 2), t3 = __shfl(t[v], base + 3);
         double a = rr_now == 0 ? t[v] : 0.0 + t[v];
-        if (nqF > 1) a = a + t1;
-        if (nqF > 2) a = a + t2;
-        if (nqF > 3) a = a + t3;
+#pragma unroll
+        for (int k = 1; k < G; k++) {
+            const double tk = __shfl(t[v], base + k);
+            if (nqF > k) a = a + tk;
+        }
         acc[v] = a;
     }
-    badL |= __shfl_xor(badL, 1);
-    badL |= __shfl_xor(badL, 2);
-    badR |= __shfl_xor(badR, 1);
-    badR |= __shfl_xor(badR, 2);
+    int bl = badL, br = badR;
+#pragma unroll
+    for (int k = 1; k < G; k++) {
+        bl |= __shfl(badL, base + k);
+        br |= __shfl(badR, base + k);
+    }
+    badL = bl;
+    badR = br;
     if (q != 0) return;
     if (recL) {
         I.faceRR[fL * fplane + nb] = (uint8_t)rr_now;
@@ -619,7 +625,7 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         }
     }
     __syncthreads();
-    const int kx = lcnt[0], ky = lcnt[1], xw = (4 * kx + 63) & ~63;
+    const int kx = lcnt[0], ky = lcnt[1];
     if (kx + ky > 0) {
         // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
         if constexpr (kPre) {
@@ -641,15 +647,19 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     __syncthreads();
     const Phys ph = phys_of(I);
     const int set = I.gpset;
-    // Four lanes per face, lane q for Gauss point q: x-faces occupy slots
-    // [0, 4 kx), y-faces [xw, xw + 4 ky) with xw a wave boundary, so each
-    // wave does faces of one direction and a sparse pass costs one point's
+    // A group of G = R + 1 lanes per face, lane q for Gauss point q, 64 / G
+    // faces to a wave (a lane or none left over): the x-faces fill the first
+    // nwx waves' worth of slots, the y-faces the next, so each wave does
+    // faces of one direction and a sparse pass costs one point's
     // reconstruction and HLLC per lane
-    for (int slot = threadIdx.x; slot < xw + 4 * ky; slot += NT) {
-        const bool isx = slot < 4 * kx;
-        if (!isx && slot < xw) continue;
-        const int q = slot & 3;
-        const uint32_t e = isx ? lst[slot >> 2] : lst[NC + ((slot - xw) >> 2)];
+    constexpr int G = gp_nq(R), FPW = 64 / G;
+    const int nwx = (kx + FPW - 1) / FPW, nwy = (ky + FPW - 1) / FPW;
+    for (int slot = threadIdx.x; slot < 64 * (nwx + nwy); slot += NT) {
+        const int w = slot >> 6, lane = slot & 63, g = lane / G, q = lane - g * G;
+        const bool isx = w < nwx;
+        const int f = (isx ? w : w - nwx) * FPW + g;
+        if (g >= FPW || f >= (isx ? kx : ky)) continue;  // the whole group skips
+        const uint32_t e = isx ? lst[f] : lst[NC + f];
         const int c = e & 0xff, lx = c % TX, ly = c / TX;
         const bool recL = e & 0x100, recR = e & 0x200, fl = e & 0x400;
         const int rr_now = (e >> 12) & 0xf, rrL = (e >> 16) & 0xf, rrR = (e >> 20) & 0xf;
@@ -657,11 +667,11 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const int cc = (ly + H) * LX + (lx + H);
         const LdsAcc<LX, LP> aR{sU + cc};
         if (isx)
-            face_quad<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
-                            q, lx, ly, bits);
+            face_group<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
+                            q, lane - q, lx, ly, bits);
         else
-            face_quad<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
-                            rrR, q, lx, ly, bits);
+            face_group<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
+                            rrR, q, lane - q, lx, ly, bits);
     }
     report_bad(I, tbad, bits);
 }
