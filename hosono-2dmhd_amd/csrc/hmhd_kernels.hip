@@ -438,9 +438,10 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
     report_bad_cells(I, bits, i, j);
 }
 
-// One face of a later MOOD iteration on a group of R + 1 lanes (the Gauss
-// points of radius R), lanes base .. base + R of the wave, lane base + q
-// for point q (src/reconstruct.f90:109-133, src/riemannSolver.f90:37-44):
+// One face of a later MOOD iteration on a group of n = rr_now + 1 lanes
+// (the face's Gauss points; one for a first-order face), lanes base ..
+// base + n - 1 of the wave, lane base + q for point q
+// (src/reconstruct.f90:109-133, src/riemannSolver.f90:37-44):
 // the lanes rebuild their point of each side to be rebuilt (at the current
 // radius) or needed for the flux (at the radius the side was last rebuilt
 // with), check its admissibility, and form w_q F_HLLC; lane q = 0 sums the
@@ -451,8 +452,8 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
 template <int R, int DIR, class A>
 __device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, int set, const A& aL, const A& aR, size_t idx,
                                           size_t nb, bool recL, bool recR, bool flux, int rr_now, int storedL,
-                                          int storedR, int q, int base, int lx, int ly, uint32_t& bits) {
-    constexpr int G = gp_nq(R);  // lanes of the group
+                                          int storedR, int q, int base, int n, int lx, int ly, uint32_t& bits) {
+    constexpr int G = gp_nq(R);  // the largest group
     const size_t fplane = (size_t)I.mx * I.my;
     constexpr int fL = DIR == 0 ? FACE_E : FACE_N, fR = DIR == 0 ? FACE_W : FACE_S;
     const int rrL = recL ? rr_now : storedL;
@@ -483,7 +484,8 @@ __device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, int se
         double a = rr_now == 0 ? t[v] : 0.0 + t[v];
 #pragma unroll
         for (int k = 1; k < G; k++) {
-            const double tk = __shfl(t[v], base + k);
+            // a lane past the group reads its own value, which is not added
+            const double tk = __shfl(t[v], k < n ? base + k : base);
             if (nqF > k) a = a + tk;
         }
         acc[v] = a;
@@ -491,8 +493,9 @@ __device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, int se
     int bl = badL, br = badR;
 #pragma unroll
     for (int k = 1; k < G; k++) {
-        bl |= __shfl(badL, base + k);
-        br |= __shfl(badR, base + k);
+        const int lk = k < n ? base + k : base;
+        bl |= __shfl(badL, lk);
+        br |= __shfl(badR, lk);
     }
     badL = bl;
     badR = br;
@@ -547,8 +550,11 @@ __device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, ui
 // change within a stage), so only the recomputation's side effects are
 // repeated: the raw-flux copy into FLB and the inadmissibility report.  The
 // remaining faces are compacted, x-faces then y-faces from a wave boundary,
-// so lanes do not idle behind untouched faces.
-template <int R>
+// so lanes do not idle behind untouched faces.  BYR: lists by radius as
+// well as direction (the multi-solve kernel, whose launches fill the chip);
+// the single-solve kernel, latency-bound on sparse tiles, keeps lists by
+// direction only.
+template <int R, bool BYR>
 __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int stage,
                                                double* __restrict__ sU, uint32_t* __restrict__ lst,
                                                int* __restrict__ lcnt) {
@@ -562,7 +568,11 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     const size_t fplane = (size_t)mx * I.my;
     const uint8_t* flag = cw.mode == MODE_NEWPASS ? I.rc : I.rec;
     uint32_t* const tbad = I.tbad + it.ty * I.ntx + it.tx;
-    if (threadIdx.x < 2) lcnt[threadIdx.x] = 0;
+    // the faces to rebuild, in lists by direction and (BYR) radius: bucket
+    // d K + rr of lst (K = R + 1 with BYR, else 1 and rr = 0; NC entries
+    // each), counts in lcnt
+    constexpr int G = gp_nq(R), K = BYR ? G : 1, NB = 2 * K;
+    if (threadIdx.x < NB) lcnt[threadIdx.x] = 0;
     if (threadIdx.x == 0) *tbad = 0;  // before the barrier, so the later ORs land on it
     __syncthreads();
 
@@ -619,14 +629,44 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
             if (recL && badL) bits |= bad_bits(d == 0 ? c % TX - 1 : c % TX, d == 0 ? c / TX : c / TX - 1);
             if (recR && badR) bits |= bad_bits(c % TX, c / TX);
         } else {
-            const int slot = atomicAdd(&lcnt[d], 1);
-            lst[d * NC + slot] = (uint32_t)c | (recL ? 0x100u : 0u) | (recR ? 0x200u : 0u) | (fl ? 0x400u : 0u) |
+            const int b = d * K + (BYR ? rr_now : 0), slot = atomicAdd(&lcnt[b], 1);
+            lst[b * NC + slot] = (uint32_t)c | (recL ? 0x100u : 0u) | (recR ? 0x200u : 0u) | (fl ? 0x400u : 0u) |
                                  (uint32_t)rr_now << 12 | (uint32_t)rrL << 16 | (uint32_t)rrR << 20;
         }
     }
     __syncthreads();
-    const int kx = lcnt[0], ky = lcnt[1];
-    if (kx + ky > 0) {
+    // Two layouts of the faces on waves.  By radius: a face of radius rr
+    // takes rr + 1 lanes, 64 / (rr + 1) faces to a wave, every bucket in
+    // waves of its own (wb[b]: the first wave of bucket b).  By direction
+    // only: every face takes R + 1 lanes (wd[d]: the first wave of
+    // direction d).  The first saves lanes where faces were lowered; it is
+    // taken unless its extra part-filled waves put another wave on one of
+    // the CU's four SIMDs (a block's waves go to the SIMDs in turn).
+    constexpr int FPWG = 64 / G, WPR = 4;
+    int wb[NB + 1], wd[3];
+    wb[0] = 0;
+    wd[0] = 0;
+    wd[1] = (lcnt[0] + FPWG - 1) / FPWG;
+    wd[2] = wd[1] + (lcnt[K] + FPWG - 1) / FPWG;
+    bool by_radius = false;
+    if constexpr (BYR) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int fpw = 64 / (b % K + 1);
+            wb[b + 1] = wb[b] + (lcnt[b] + fpw - 1) / fpw;
+        }
+        wd[1] = 0;
+#pragma unroll
+        for (int r = 0; r < G; r++) wd[1] += lcnt[r];
+        wd[2] = 0;
+#pragma unroll
+        for (int r = 0; r < G; r++) wd[2] += lcnt[G + r];
+        wd[1] = (wd[1] + FPWG - 1) / FPWG;
+        wd[2] = wd[1] + (wd[2] + FPWG - 1) / FPWG;
+        by_radius = (wb[NB] + WPR - 1) / WPR <= (wd[2] + WPR - 1) / WPR;
+    }
+    const int nwaves = by_radius ? wb[NB] : wd[2];
+    if (nwaves > 0) {
         // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
         if constexpr (kPre) {
             if (threadIdx.x < LP) {
@@ -647,19 +687,34 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     __syncthreads();
     const Phys ph = phys_of(I);
     const int set = I.gpset;
-    // A group of G = R + 1 lanes per face, lane q for Gauss point q, 64 / G
-    // faces to a wave (a lane or none left over): the x-faces fill the first
-    // nwx waves' worth of slots, the y-faces the next, so each wave does
-    // faces of one direction and a sparse pass costs one point's
+    // A group of n lanes per face, lane q for Gauss point q (n = rr + 1 by
+    // radius, R + 1 by direction): every wave does faces of one direction
+    // (and, by radius, of one radius), and a sparse pass costs one point's
     // reconstruction and HLLC per lane
-    constexpr int G = gp_nq(R), FPW = 64 / G;
-    const int nwx = (kx + FPW - 1) / FPW, nwy = (ky + FPW - 1) / FPW;
-    for (int slot = threadIdx.x; slot < 64 * (nwx + nwy); slot += NT) {
-        const int w = slot >> 6, lane = slot & 63, g = lane / G, q = lane - g * G;
-        const bool isx = w < nwx;
-        const int f = (isx ? w : w - nwx) * FPW + g;
-        if (g >= FPW || f >= (isx ? kx : ky)) continue;  // the whole group skips
-        const uint32_t e = isx ? lst[f] : lst[NC + f];
+    for (int slot = threadIdx.x; slot < 64 * nwaves; slot += NT) {
+        const int w = slot >> 6, lane = slot & 63;
+        int b, n, f, g;
+        if (by_radius) {
+            b = 0;
+            while (w >= wb[b + 1]) b++;  // the same for the whole wave
+            n = b % K + 1;
+            g = lane / n;
+            f = (w - wb[b]) * (64 / n) + g;
+            if (g >= 64 / n || f >= lcnt[b]) continue;  // the whole group skips
+        } else {
+            const int d = w >= wd[1] ? 1 : 0;
+            n = G;
+            g = lane / G;
+            f = (w - wd[d]) * FPWG + g;
+            if (g >= FPWG) continue;
+            b = d * K;
+            if constexpr (BYR)
+                while (b < d * K + K - 1 && f >= lcnt[b]) f -= lcnt[b++];  // the radius bucket of face f
+            if (f >= lcnt[b]) continue;  // past the direction's last face: the whole group skips
+        }
+        const int q = lane - g * n;
+        const bool isx = b < K;
+        const uint32_t e = lst[b * NC + f];
         const int c = e & 0xff, lx = c % TX, ly = c / TX;
         const bool recL = e & 0x100, recR = e & 0x200, fl = e & 0x400;
         const int rr_now = (e >> 12) & 0xf, rrL = (e >> 16) & 0xf, rrR = (e >> 20) & 0xf;
@@ -668,10 +723,10 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const LdsAcc<LX, LP> aR{sU + cc};
         if (isx)
             face_group<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
-                            q, lane - q, lx, ly, bits);
+                            q, lane - q, n, lx, ly, bits);
         else
             face_group<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
-                            rrR, q, lane - q, lx, ly, bits);
+                            rrR, q, lane - q, n, lx, ly, bits);
     }
     report_bad(I, tbad, bits);
 }
@@ -681,15 +736,15 @@ __global__ __launch_bounds__(face_threads(R, false), R >= 3 ? 1 : 2) void k_face
                                                                                  const Inst one, const TileList tl,
                                                                                  int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
-    __shared__ uint32_t lst[2 * kFaceTX * kFaceTY];
-    __shared__ int lcnt[2];
+    __shared__ uint32_t lst[2 * (ONE ? 1 : gp_nq(R)) * kFaceTX * kFaceTY];
+    __shared__ int lcnt[2 * (ONE ? 1 : gp_nq(R))];
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
     bool later = false;
     for_tiles(tl, [&](const Item it) {
         if (later) __syncthreads();  // the previous tile's LDS reads are done
         later = true;
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
-        face_tile_iter<R>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
+        face_tile_iter<R, !ONE>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
     }, c1.run | c1.mode);
 }
 
