@@ -78,19 +78,54 @@ __device__ __forceinline__ const double* pv_ptr(int set, int rr, int f, int q) {
     return &c_pv[set][gp_pv_off(rr) + (f * gp_nq(rr) + q) * gp_ns(rr)];
 }
 
+// gp_point for the later passes: the weights and stencil values of CH
+// cells are read from LDS together, then summed in stencil order (the same
+// operations in the same order).  Left to itself, the compiler, under the
+// register pressure of the later-pass face kernel, waits for each read
+// before the next, a round trip per read.
+template <int RR, int CH, class A>
+__device__ __forceinline__ void gp_point_chunked(const A& a, const double* __restrict__ w, double* out) {
+    constexpr Sten<RR> st{};
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) out[v] = 0.0;
+#pragma unroll
+    for (int k0 = 0; k0 < st.n; k0 += CH) {
+        double x[CH][NCONS], wk[CH];
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            if (k0 + c < st.n) {
+                wk[c] = w[k0 + c];
+#pragma unroll
+                for (int v = 0; v < NCONS; v++) x[c][v] = a.at(v, st.x[k0 + c], st.y[k0 + c]);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            if (k0 + c < st.n) {
+#pragma unroll
+                for (int v = 0; v < NCONS; v++) out[v] = out[v] + wk[c] * x[c][v];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // value of one Riemann state at point q from a side reconstructed with
-// radius rr (rr == 0: the cell average itself)
+// radius rr (rr == 0: the cell average itself); pv: a copy of the solve's
+// table c_pv[set] (in LDS)
 template <int R, class A>
-__device__ __forceinline__ void side_value(const A& a, int set, int rr, int f, int q, double* out) {
+__device__ __forceinline__ void side_value(const A& a, const double* pv, int rr, int f, int q, double* out) {
+    constexpr int CH = 5;
     if (rr == 0) {
 #pragma unroll
         for (int v = 0; v < NCONS; v++) out[v] = a.at(v, 0, 0);
     } else if (rr == R) {
-        gp_point<R>(a, pv_ptr(set, R, f, q), out);
+        gp_point_chunked<R, CH>(a, pv + gp_pv_off(R) + (f * gp_nq(R) + q) * gp_ns(R), out);
     } else if (R >= 3 && rr == 2) {
-        gp_point<(R >= 3 ? 2 : 1)>(a, pv_ptr(set, R >= 3 ? 2 : 1, f, q), out);
+        gp_point_chunked<(R >= 3 ? 2 : 1), CH>(a, pv + gp_pv_off(2) + (f * gp_nq(2) + q) * gp_ns(2), out);
     } else {
-        gp_point<1>(a, pv_ptr(set, 1, f, q), out);
+        gp_point_chunked<1, CH>(a, pv + gp_pv_off(1) + (f * gp_nq(1) + q) * gp_ns(1), out);
     }
 }
 
@@ -448,9 +483,11 @@ __device__ __forceinline__ void face_tile_full(const Inst& I, int i0, int j0, in
 // points in ascending order from zero (or takes the one first-order flux
 // as it is) and writes the face's results.
 // rr_now: the face's radius, min of its cells' schemes; storedL/R: the
-// radius each side was last reconstructed with.
+// radius each side was last reconstructed with; pv, qw: the solve's point
+// and quadrature weights (c_pv[set], c_qw[set]) as copied to LDS.
 template <int R, int DIR, class A>
-__device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, int set, const A& aL, const A& aR, size_t idx,
+__device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, const double* pv, const double* qw,
+                                          const A& aL, const A& aR, size_t idx,
                                           size_t nb, bool recL, bool recR, bool flux, int rr_now, int storedL,
                                           int storedR, int q, int base, int n, int lx, int ly, uint32_t& bits) {
     constexpr int G = gp_nq(R);  // the largest group
@@ -463,14 +500,14 @@ __device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, int se
     const int nqF = flux ? (rr_now == 0 ? 1 : gp_nq(rr_now)) : 0;
     double uL[NCONS], uR[NCONS], t[NCONS];
     int badL = 0, badR = 0;
-    if (q < nqL || q < nqF) side_value<R>(aL, set, rrL, fL, q, uL);
-    if (q < nqR || q < nqF) side_value<R>(aR, set, rrR, fR, q, uR);
+    if (q < nqL || q < nqF) side_value<R>(aL, pv, rrL, fL, q, uL);
+    if (q < nqR || q < nqF) side_value<R>(aR, pv, rrR, fR, q, uR);
     if (q < nqL) badL = !admissible(uL[DENS], cons2pres(ph, uL));
     if (q < nqR) badR = !admissible(uR[DENS], cons2pres(ph, uR));
     if (q < nqF) {
         double F[NCONS];
         hllc<DIR>(ph, uL, uR, F);
-        const double w = c_qw[set][rr_now][q];
+        const double w = qw[rr_now * 4 + q];
 #pragma unroll
         for (int v = 0; v < NCONS; v++) t[v] = rr_now == 0 ? F[v] : w * F[v];
     } else {
@@ -557,7 +594,8 @@ __device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, ui
 template <int R, bool BYR>
 __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int stage,
                                                double* __restrict__ sU, uint32_t* __restrict__ lst,
-                                               int* __restrict__ lcnt) {
+                                               int* __restrict__ lcnt, double* __restrict__ sPV,
+                                               double* __restrict__ sQW) {
     constexpr int TX = kFaceTX, TY = kFaceTY, NC = TX * TY, H = R + 1, NT = face_threads(R, false);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
@@ -585,6 +623,19 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     const size_t uplane = (size_t)I.nx * I.ny;
     double pre[NCONS];
     uint32_t bits = 0;
+    // The solve's GP weights for radii 1..R and its quadrature weights, for
+    // LDS: a lane's weights depend on its face's radius and Gauss point, so
+    // read from the constant tables they would be a dependent global load
+    // per side inside the rebuild; loaded here they arrive with the
+    // classification's values.
+    constexpr int NPV = gp_pv_off(R + 1), NQW = (R + 1) * 4, KW = (NPV + NQW + NT - 1) / NT;
+    const int set = I.gpset;
+    double wv[KW];
+#pragma unroll
+    for (int k = 0; k < KW; k++) {
+        const int e = threadIdx.x + k * NT;
+        wv[k] = e < NPV ? c_pv[set][e] : (e < NPV + NQW ? (&c_qw[set][0][0])[e - NPV] : 0.0);
+    }
     for (int f = threadIdx.x; f < 2 * NC; f += NT) {
         const int d = f / NC, c = f - d * NC;
         const int i = i0 + c % TX, j = j0 + c / TX;
@@ -633,6 +684,14 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
             lst[b * NC + slot] = (uint32_t)c | (recL ? 0x100u : 0u) | (recR ? 0x200u : 0u) | (fl ? 0x400u : 0u) |
                                  (uint32_t)rr_now << 12 | (uint32_t)rrL << 16 | (uint32_t)rrR << 20;
         }
+    }
+#pragma unroll
+    for (int k = 0; k < KW; k++) {
+        const int e = threadIdx.x + k * NT;
+        if (e < NPV)
+            sPV[e] = wv[k];
+        else if (e < NPV + NQW)
+            sQW[e - NPV] = wv[k];
     }
     __syncthreads();
     // Two layouts of the faces on waves.  By radius: a face of radius rr
@@ -686,7 +745,6 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     }
     __syncthreads();
     const Phys ph = phys_of(I);
-    const int set = I.gpset;
     // A group of n lanes per face, lane q for Gauss point q (n = rr + 1 by
     // radius, R + 1 by direction): every wave does faces of one direction
     // (and, by radius, of one radius), and a sparse pass costs one point's
@@ -722,10 +780,10 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const int cc = (ly + H) * LX + (lx + H);
         const LdsAcc<LX, LP> aR{sU + cc};
         if (isx)
-            face_group<R, 0>(I, ph, set, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
+            face_group<R, 0>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
                             q, lane - q, n, lx, ly, bits);
         else
-            face_group<R, 1>(I, ph, set, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
+            face_group<R, 1>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
                             rrR, q, lane - q, n, lx, ly, bits);
     }
     report_bad(I, tbad, bits);
@@ -738,13 +796,14 @@ __global__ __launch_bounds__(face_threads(R, false), R >= 3 ? 1 : 2) void k_face
     __shared__ double sU[NCONS * face_lds_cells<R>()];
     __shared__ uint32_t lst[2 * (ONE ? 1 : gp_nq(R)) * kFaceTX * kFaceTY];
     __shared__ int lcnt[2 * (ONE ? 1 : gp_nq(R))];
+    __shared__ double sPV[gp_pv_off(R + 1)], sQW[(R + 1) * 4];
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
     bool later = false;
     for_tiles(tl, [&](const Item it) {
         if (later) __syncthreads();  // the previous tile's LDS reads are done
         later = true;
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
-        face_tile_iter<R, !ONE>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt);
+        face_tile_iter<R, !ONE>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt, sPV, sQW);
     }, c1.run | c1.mode);
 }
 
@@ -848,41 +907,60 @@ __device__ __forceinline__ bool band_cell(const Inst& I, int i, int j) {
     return i < s2 || i >= I.nx || j < s2 || j >= I.ny;
 }
 
+// The window of `low` within four cells of a tile, [x0-4, x0+kCellT+4) x
+// [y0-4, y0+kCellT+4), row pitch kWinW: byte lane + 64 r of it in v[r], 0
+// off the grid (clamped, masked loads, all issued at once).
+constexpr int kWinW = kCellT + 8, kWinN = (kWinW * kWinW + 63) / 64;
+__device__ __forceinline__ void low_window(const Inst& I, int x0, int y0, int lane, uint8_t (&v)[kWinN]) {
+#pragma unroll
+    for (int r = 0; r < kWinN; r++) {
+        const int c = lane + r * 64, i = x0 - 4 + c % kWinW, j = y0 - 4 + c / kWinW;
+        const bool in = c < kWinW * kWinW && i >= 0 && j >= 0 && i < I.mx && j < I.my;
+        const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
+        v[r] = in ? I.low[(size_t)cj * I.mx + ci] : 0;
+    }
+}
+
 // The post-detect lowering (src/detect.f90:60-72,219-234) of the cells
 // lane, lane + 64, lane + 128, lane + 192 of one tile, as a gather: rc =
-// cross(low), rcF = diamond(low), the scheme lowered where low.  Every load
-// of the four cells is issued before the first store.  Band cells are left
-// to the sync when `skip_bands` (a periodic solve whose pass syncs).
-__device__ __forceinline__ void lower_detect_wave(const Inst& I, const Item it, int lane, bool skip_bands) {
-    constexpr int NC = kCellT * kCellT / 64;
-    uint8_t own[NC], cross[NC], diam[NC], sc[NC];
-    bool on[NC];
-    size_t idx[NC];
+// cross(low), rcF = diamond(low), the scheme lowered where low.  load()
+// reads the cells' scheme; store() gathers cross and diamond from the
+// tile's window of low (low_window, in LDS) and writes.  Band cells are
+// left to the sync when `skip_bands` (a periodic solve whose pass syncs).
+struct LowerDetect {
+    static constexpr int NC = kCellT * kCellT / 64;
+    uint8_t sc[NC];
+    __device__ __forceinline__ void load(const Inst& I, const Item it, int lane);
+    __device__ __forceinline__ void store(const Inst& I, const Item it, int lane, const uint8_t* win,
+                                          bool skip_bands) const;
+};
+__device__ __forceinline__ void LowerDetect::load(const Inst& I, const Item it, int lane) {
 #pragma unroll
     for (int r = 0; r < NC; r++) {
         const int c = lane + 64 * r, i = it.tx * kCellT + c % kCellT, j = it.ty * kCellT + c / kCellT;
-        on[r] = i < I.mx && j < I.my && !(skip_bands && band_cell(I, i, j));
         const int ci = i < I.mx ? i : I.mx - 1, cj = j < I.my ? j : I.my - 1;
-        idx[r] = (size_t)cj * I.mx + ci;
-        const Nbhd nb = nbhd(ci, cj, I.mx, I.my);
+        sc[r] = I.scheme[(size_t)cj * I.mx + ci];
+    }
+}
+__device__ __forceinline__ void LowerDetect::store(const Inst& I, const Item it, int lane, const uint8_t* win,
+                                                   bool skip_bands) const {
+#pragma unroll
+    for (int r = 0; r < NC; r++) {
+        const int c = lane + 64 * r, lx = c % kCellT, ly = c / kCellT;
+        const int i = it.tx * kCellT + lx, j = it.ty * kCellT + ly;
+        if (!(i < I.mx && j < I.my && !(skip_bands && band_cell(I, i, j)))) continue;
+        const uint8_t* w = win + (ly + 4) * kWinW + (lx + 4);
         uint8_t cr = 0, dm = 0;
 #pragma unroll
         for (int k = 0; k < 13; k++) {
-            const uint8_t x = nb.in[k] ? I.low[nb.off[k]] : 0;
-            if (k == 0) own[r] = x;
+            const uint8_t x = w[kDiamDy[k] * kWinW + kDiamDx[k]];
             if (k < 5) cr |= x;
             dm |= x;
         }
-        cross[r] = cr;
-        diam[r] = dm;
-        sc[r] = I.scheme[idx[r]];
-    }
-#pragma unroll
-    for (int r = 0; r < NC; r++) {
-        if (!on[r]) continue;
-        I.rc[idx[r]] = cross[r];
-        I.rcF[idx[r]] = diam[r];
-        if (own[r]) I.scheme[idx[r]] = (uint8_t)(sc[r] > 0 ? sc[r] - 1 : 0);
+        const size_t idx = (size_t)j * I.mx + i;
+        I.rc[idx] = cr;
+        I.rcF[idx] = dm;
+        if (w[0]) I.scheme[idx] = (uint8_t)(sc[r] > 0 ? sc[r] - 1 : 0);
     }
 }
 
@@ -1326,18 +1404,11 @@ constexpr int kScanTiles = 4;
 
 // whether a low cell lies within four cells of the tile at (x0, y0)
 __device__ __forceinline__ bool low_near(const Inst& I, int x0, int y0, int lane) {
-    constexpr int W = kCellT + 8, NR = (W * W + 63) / 64;
-    uint8_t v[NR];
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int c = lane + r * 64, i = x0 - 4 + c % W, j = y0 - 4 + c / W;
-        const bool in = c < W * W && i >= 0 && j >= 0 && i < I.mx && j < I.my;
-        const int ci = i < 0 ? 0 : (i >= I.mx ? I.mx - 1 : i), cj = j < 0 ? 0 : (j >= I.my ? I.my - 1 : j);
-        v[r] = in ? I.low[(size_t)cj * I.mx + ci] : 0;
-    }
+    uint8_t v[kWinN];
+    low_window(I, x0, y0, lane, v);
     uint8_t o = 0;
 #pragma unroll
-    for (int r = 0; r < NR; r++) o |= v[r];
+    for (int r = 0; r < kWinN; r++) o |= v[r];
     return o != 0;
 }
 
@@ -1355,6 +1426,7 @@ __global__ __launch_bounds__(64 * kScanTiles, 1) void k_end_iter(const Inst* __r
                                                               int sync_gx) {
     __shared__ int hit[kScanTiles], hitf[kScanTiles];
     __shared__ int base, basef;
+    __shared__ uint8_t swin[kScanTiles][kWinW * kWinW];
     const int nsync = sync_gx * ninst, nfin = snap != nullptr ? 1 : 0;
     if ((int)blockIdx.x < nsync) {
         // the periodic MOOD sync, sync_gx blocks per solve
@@ -1401,15 +1473,36 @@ __global__ __launch_bounds__(64 * kScanTiles, 1) void k_end_iter(const Inst* __r
     // the atomic (with one per block, a full list cost as much as all the
     // rest of the scan)
     const bool idlist = ONE && !ready && I.ctl[CTL_FIRST] != 0;
-    if (t < npad && ready && (stamp == 0 || amarks[tile] == stamp)) lower_detect_wave(I, it, lane, sync);
+    // One round of loads for what the wave does: the window of low within
+    // four cells of the tile (the scan's test after a ready pass and,
+    // through LDS, the gather of the lowering after detect), the tile's
+    // scheme, and, in place of the inadmissible sides, the bad_bits of the
+    // 3 x 3 tiles around (every inadmissible side of a footprint cell was
+    // written by one of them).  Nothing in this launch writes low or tbad.
+    const bool dolow = t < npad && ready && (stamp == 0 || amarks[tile] == stamp);
+    const bool scanned = t < npad && (idlist || stamp == 0 || marks[tile] == stamp);
+    const int X0 = it.tx * kCellT, Y0 = it.ty * kCellT;
+    uint8_t win[kWinN];
+    if (ready && (dolow || scanned)) low_window(I, X0, Y0, lane, win);
+    LowerDetect ld;
+    if (dolow) ld.load(I, it, lane);
+    const int nty = (I.my + kCellT - 1) / kCellT, ta = it.tx + lane % 3 - 1, tb = it.ty + lane / 3 - 1;
+    const bool tin = lane < 9 && ta >= 0 && tb >= 0 && ta < I.ntx && tb < nty;
+    const uint32_t tf = scanned ? I.tbad[(tin ? tb : it.ty) * I.ntx + (tin ? ta : it.tx)] : 0u;
+    if (dolow) {
+        // the window to this wave's LDS rows (a wave's LDS accesses complete
+        // in order), then the lowering
+        uint8_t* sw = swin[w];
+#pragma unroll
+        for (int r = 0; r < kWinN; r++) {
+            const int c = lane + r * 64;
+            if (c < kWinW * kWinW) sw[c] = win[r];
+        }
+        __builtin_amdgcn_wave_barrier();
+        ld.store(I, it, lane, sw, sync);
+    }
     bool any = false, anyf = false;
-    if (t < npad && (idlist || stamp == 0 || marks[tile] == stamp)) {
-        // in place of the inadmissible sides, the bad_bits of the 3 x 3
-        // tiles around: every inadmissible side of a footprint cell was
-        // written by one of them
-        const int nty = (I.my + kCellT - 1) / kCellT, ta = it.tx + lane % 3 - 1, tb = it.ty + lane / 3 - 1;
-        const bool tin = lane < 9 && ta >= 0 && tb >= 0 && ta < I.ntx && tb < nty;
-        const uint32_t tf = I.tbad[(tin ? tb : it.ty) * I.ntx + (tin ? ta : it.tx)];
+    if (scanned) {
         // a neighbour's sides count when they lie within two cells of this
         // tile: the edge (or corner) of the neighbour facing it
         const int dx = lane % 3 - 1, dy = lane / 3 - 1;
@@ -1418,9 +1511,11 @@ __global__ __launch_bounds__(64 * kScanTiles, 1) void k_end_iter(const Inst* __r
         const uint32_t need4 = BAD_ANY | (dx < 0 ? BAD_E4 : 0u) | (dx > 0 ? BAD_W4 : 0u) | (dy < 0 ? BAD_N4 : 0u) |
                                (dy > 0 ? BAD_S4 : 0u);
         anyf = tin && (tf & need2) == need2;
-        const int X0 = it.tx * kCellT, Y0 = it.ty * kCellT;
         if (ready) {
-            any = anyf || low_near(I, X0, Y0, lane);
+            uint8_t o = 0;
+#pragma unroll
+            for (int r = 0; r < kWinN; r++) o |= win[r];
+            any = anyf || o != 0;
             // a band cell of the footprint ends the launch with the OR over
             // its sync unit: its images' low cells count too
             if (sync && !__ballot(any)) {
