@@ -555,6 +555,109 @@ __device__ __forceinline__ void face_group(const Inst& I, const Phys& ph, const 
     }
 }
 
+// x from the other lane of this lane's pair (lanes 2k, 2k + 1), by DPP;
+// both lanes must be active
+__device__ __forceinline__ double pair_swap(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double((long long)hi << 32 | (unsigned int)lo);
+}
+
+// One face of a later MOOD iteration on 2 (R + 1) lanes, lane base + 2q + sd
+// for Gauss point q of side sd (0: L, 1: R): face_group with the two sides
+// on a pair of lanes instead of one lane, for the single-solve kernel,
+// whose sparse passes wait on the dependency chain of one face.  Each lane
+// rebuilds its side at its point and checks its admissibility and forms
+// the side's part of HLLC (hllc_side); the pair exchanges states and side
+// parts, and lane 2q finishes HLLC with its own side as L (the R lane's
+// result is not used).  Lane base sums the points in ascending order from
+// zero and writes the face's results, as in face_group.
+template <int R, int DIR, class A>
+__device__ __forceinline__ void face_group_pair(const Inst& I, const Phys& ph, const double* pv, const double* qw,
+                                               const A& aS, size_t idx, size_t nb, bool recL, bool recR, bool flux,
+                                               int rr_now, int storedL, int storedR, int q, int sd, int base, int lx,
+                                               int ly, uint32_t& bits) {
+    constexpr int G = gp_nq(R);
+    const size_t fplane = (size_t)I.mx * I.my;
+    constexpr int fL = DIR == 0 ? FACE_E : FACE_N, fR = DIR == 0 ? FACE_W : FACE_S;
+    const int rrL = recL ? rr_now : storedL;
+    const int rrR = recR ? rr_now : storedR;
+    const int nqL = (recL && rrL > 0) ? gp_nq(rrL) : 0;
+    const int nqR = (recR && rrR > 0) ? gp_nq(rrR) : 0;
+    const int nqF = flux ? (rr_now == 0 ? 1 : gp_nq(rr_now)) : 0;
+    const int rrS = sd ? rrR : rrL, nqS = sd ? nqR : nqL;
+    double u[NCONS], t[NCONS];
+    int bad = 0;
+    if (q < nqS || q < nqF) side_value<R>(aS, pv, rrS, sd ? fR : fL, q, u);
+    if (q < nqS) bad = !admissible(u[DENS], cons2pres(ph, u));
+    if (q < nqF) {
+        // both lanes of the pair are here (q and the face are theirs alike)
+        const HSide h = hllc_side<DIR>(ph, u);
+        HSide ho;
+        double uo[NCONS];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) {
+            uo[v] = pair_swap(u[v]);
+            ho.f[v] = pair_swap(h.f[v]);
+        }
+        ho.V.d = pair_swap(h.V.d);
+        ho.V.vx = pair_swap(h.V.vx);
+        ho.V.vy = pair_swap(h.V.vy);
+        ho.V.vz = pair_swap(h.V.vz);
+        ho.V.p = pair_swap(h.V.p);
+        ho.V.bx = pair_swap(h.V.bx);
+        ho.V.by = pair_swap(h.V.by);
+        ho.V.bz = pair_swap(h.V.bz);
+        ho.c = pair_swap(h.c);
+        ho.mdv = pair_swap(h.mdv);
+        ho.pTot = pair_swap(h.pTot);
+        double F[NCONS];
+        hllc_from_sides<DIR>(ph, u, uo, h, ho, F);
+        const double w = qw[rr_now * 4 + q];
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) t[v] = rr_now == 0 ? F[v] : w * F[v];
+    } else {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) t[v] = 0.0;
+    }
+    // every lane of the group takes part in the exchanges
+    double acc[NCONS];
+#pragma unroll
+    for (int v = 0; v < NCONS; v++) {
+        double a = rr_now == 0 ? t[v] : 0.0 + t[v];
+#pragma unroll
+        for (int k = 1; k < G; k++) {
+            const double tk = __shfl(t[v], base + 2 * k);
+            if (nqF > k) a = a + tk;
+        }
+        acc[v] = a;
+    }
+    int badL = 0, badR = 0;
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+        badL |= __shfl(bad, base + 2 * k);
+        badR |= __shfl(bad, base + 2 * k + 1);
+    }
+    if (q != 0 || sd != 0) return;
+    if (recL) {
+        I.faceRR[fL * fplane + nb] = (uint8_t)rr_now;
+        I.inadm[fL * fplane + nb] = (uint8_t)badL;
+    }
+    if (recR) {
+        I.faceRR[fR * fplane + idx] = (uint8_t)rr_now;
+        I.inadm[fR * fplane + idx] = (uint8_t)badR;
+    }
+    if (recL && badL) bits |= bad_bits(DIR == 0 ? lx - 1 : lx, DIR == 0 ? ly : ly - 1);
+    if (recR && badR) bits |= bad_bits(lx, ly);
+    if (flux) {
+#pragma unroll
+        for (int v = 0; v < NCONS; v++) I.FR[(DIR * NCONS + v) * fplane + idx] = acc[v];
+        I.FLB[DIR * fplane + idx] = acc[DIR == 0 ? MAGY : MAGX];
+        I.fluxRR[DIR * fplane + idx] = (uint8_t)rr_now;
+    }
+}
+
 // Tiles whose flags this iteration may change, for the scan that ends it:
 // within one tile of an active tile (every kernel writes flags of its own
 // tile's cells, the face kernel also the east / north sides of the cells
@@ -701,7 +804,9 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     // direction d).  The first saves lanes where faces were lowered; it is
     // taken unless its extra part-filled waves put another wave on one of
     // the CU's four SIMDs (a block's waves go to the SIMDs in turn).
-    constexpr int FPWG = 64 / G, WPR = 4;
+    // By direction, the multi-solve kernel gives a face R + 1 lanes
+    // (face_group), the single-solve kernel 2 (R + 1) (face_group_pair).
+    constexpr int PG = BYR ? G : 2 * G, FPWG = 64 / PG, WPR = 4;
     int wb[NB + 1], wd[3];
     wb[0] = 0;
     wd[0] = 0;
@@ -761,8 +866,8 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
             if (g >= 64 / n || f >= lcnt[b]) continue;  // the whole group skips
         } else {
             const int d = w >= wd[1] ? 1 : 0;
-            n = G;
-            g = lane / G;
+            n = PG;
+            g = lane / PG;
             f = (w - wd[d]) * FPWG + g;
             if (g >= FPWG) continue;
             b = d * K;
@@ -770,7 +875,7 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
                 while (b < d * K + K - 1 && f >= lcnt[b]) f -= lcnt[b++];  // the radius bucket of face f
             if (f >= lcnt[b]) continue;  // past the direction's last face: the whole group skips
         }
-        const int q = lane - g * n;
+        const int r = lane - g * n;
         const bool isx = b < K;
         const uint32_t e = lst[b * NC + f];
         const int c = e & 0xff, lx = c % TX, ly = c / TX;
@@ -779,12 +884,23 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const size_t idx = (size_t)(j0 + ly) * mx + (i0 + lx);
         const int cc = (ly + H) * LX + (lx + H);
         const LdsAcc<LX, LP> aR{sU + cc};
-        if (isx)
-            face_group<R, 0>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now, rrL, rrR,
-                            q, lane - q, n, lx, ly, bits);
-        else
-            face_group<R, 1>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now, rrL,
-                            rrR, q, lane - q, n, lx, ly, bits);
+        if constexpr (BYR) {
+            if (isx)
+                face_group<R, 0>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now,
+                                rrL, rrR, r, lane - r, n, lx, ly, bits);
+            else
+                face_group<R, 1>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now,
+                                rrL, rrR, r, lane - r, n, lx, ly, bits);
+        } else {
+            // lane r of the group: point r / 2, side r % 2 (1: R, the cell itself)
+            const int q = r >> 1, sd = r & 1;
+            if (isx)
+                face_group_pair<R, 0>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - (sd ? 0 : 1)}, idx, idx - 1, recL, recR,
+                                      fl, rr_now, rrL, rrR, q, sd, lane - r, lx, ly, bits);
+            else
+                face_group_pair<R, 1>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - (sd ? 0 : LX)}, idx, idx - mx, recL,
+                                      recR, fl, rr_now, rrL, rrR, q, sd, lane - r, lx, ly, bits);
+        }
     }
     report_bad(I, tbad, bits);
 }

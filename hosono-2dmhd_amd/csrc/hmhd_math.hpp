@@ -172,11 +172,32 @@ __device__ __forceinline__ void cons2flux(const Phys& ph, const double* U, const
     }
 }
 
+// What HLLC computes from one side's state alone: its primitives, fast
+// speed and physical flux, B.v and the total pressure.
+struct HSide {
+    Prim V;
+    double c, mdv, pTot;
+    double f[NCONS];
+};
 template <int DIR>
-__device__ __forceinline__ void hllc(const Phys& ph, const double* uL, const double* uR, double* flux) {
+__device__ __forceinline__ HSide hllc_side(const Phys& ph, const double* u) {
+    HSide h;
+    h.V = cons2prim(ph, u);
+    h.c = fast_speed<DIR>(ph, h.V);
+    cons2flux<DIR>(ph, u, h.V, h.f);
+    h.mdv = h.V.bx * h.V.vx + h.V.by * h.V.vy + h.V.bz * h.V.vz;
+    h.pTot = h.V.p + 0.5 * (h.V.bx * h.V.bx + h.V.by * h.V.by + h.V.bz * h.V.bz);
+    return h;
+}
+
+// HLLC-MHD (src/riemannSolver.f90:68-221) from the two sides' states and
+// their hllc_side values; hllc() below is both together.
+template <int DIR>
+__device__ __forceinline__ void hllc_from_sides(const Phys& ph, const double* uL, const double* uR, const HSide& hL,
+                                                const HSide& hR, double* flux) {
     using D = Dir<DIR>;
-    const Prim vL = cons2prim(ph, uL);
-    const Prim vR = cons2prim(ph, uR);
+    const Prim& vL = hL.V;
+    const Prim& vR = hR.V;
     Prim vA;
     vA.d = (vL.d + vR.d) / 2.0;
     vA.vx = (vL.vx + vR.vx) / 2.0;
@@ -186,22 +207,22 @@ __device__ __forceinline__ void hllc(const Phys& ph, const double* uL, const dou
     vA.bx = (vL.bx + vR.bx) / 2.0;
     vA.by = (vL.by + vR.by) / 2.0;
     vA.bz = (vL.bz + vR.bz) / 2.0;
-    const double cL = fast_speed<DIR>(ph, vL);
-    const double cR = fast_speed<DIR>(ph, vR);
+    const double cL = hL.c;
+    const double cR = hR.c;
     const double cA = fast_speed<DIR>(ph, vA);
     const double vNL = pcomp(vL, D::mN), vNR = pcomp(vR, D::mN), vNA = pcomp(vA, D::mN);
     const double sL = fminF(vNL - cL, vNA - cA);
     const double sR = fmaxF(vNA + cA, vNR + cR);
 
-    double fL[NCONS], fR[NCONS], uHLL[NCONS];
-    cons2flux<DIR>(ph, uL, vL, fL);
-    cons2flux<DIR>(ph, uR, vR, fR);
+    const double* fL = hL.f;
+    const double* fR = hR.f;
+    double uHLL[NCONS];
 #pragma unroll
     for (int v = 0; v < NCONS; v++) uHLL[v] = (sR * uR[v] - sL * uL[v] + fL[v] - fR[v]) / (sR - sL);
     const double velx = uHLL[MOMX] / uHLL[DENS], vely = uHLL[MOMY] / uHLL[DENS], velz = uHLL[MOMZ] / uHLL[DENS];
     const double mdvHLL = uHLL[MAGX] * velx + uHLL[MAGY] * vely + uHLL[MAGZ] * velz;
-    const double mdvL = vL.bx * vL.vx + vL.by * vL.vy + vL.bz * vL.vz;
-    const double mdvR = vR.bx * vR.vx + vR.by * vR.vy + vR.bz * vR.vz;
+    const double mdvL = hL.mdv;
+    const double mdvR = hR.mdv;
 
     const double velT1L = pcomp(vL, D::mT1), velT1R = pcomp(vR, D::mT1);
     const double velT2L = pcomp(vL, D::mT2), velT2R = pcomp(vR, D::mT2);
@@ -210,8 +231,8 @@ __device__ __forceinline__ void hllc(const Phys& ph, const double* uL, const dou
     const double magT1L = pcomp(vL, D::bT1), magT1R = pcomp(vR, D::bT1), magT1HLL = uHLL[D::bT1];
     const double magT2L = pcomp(vL, D::bT2), magT2R = pcomp(vR, D::bT2), magT2HLL = uHLL[D::bT2];
     const double rhoL = vL.d, rhoR = vR.d;
-    const double pTotL = vL.p + 0.5 * (vL.bx * vL.bx + vL.by * vL.by + vL.bz * vL.bz);
-    const double pTotR = vR.p + 0.5 * (vR.bx * vR.bx + vR.by * vR.by + vR.bz * vR.bz);
+    const double pTotL = hL.pTot;
+    const double pTotR = hR.pTot;
     const double dL = sL - vNL, dR = sR - vNR;
     const double sStr =
         (momNR * dR - momNL * dL + pTotL - pTotR - magNL * magNL + magNR * magNR) / (rhoR * dR - rhoL * dL);
@@ -250,6 +271,13 @@ __device__ __forceinline__ void hllc(const Phys& ph, const double* uL, const dou
 #pragma unroll
         for (int v = 0; v < NCONS; v++) flux[v] = fR[v];
     }
+}
+
+template <int DIR>
+__device__ __forceinline__ void hllc(const Phys& ph, const double* uL, const double* uR, double* flux) {
+    const HSide hL = hllc_side<DIR>(ph, uL);
+    const HSide hR = hllc_side<DIR>(ph, uR);
+    hllc_from_sides<DIR>(ph, uL, uR, hL, hR, flux);
 }
 
 
