@@ -208,6 +208,33 @@ __device__ __forceinline__ void for_tiles(const TileList& tl, F&& body, int pin 
     }
 }
 
+// The later-pass update runs a tile as kUpdQ blocks of one wave, kUpdRows
+// rows of the tile each: a lone wave's loads are bound by the cache lines
+// its CU has in flight, so a tile's loads spread over four CUs finish
+// sooner than over one.
+constexpr int kUpdRows = 4, kUpdQ = kCellT / kUpdRows;
+static_assert(kUpdRows * kCellT == 64, "one wave per quarter tile");
+
+// for_tiles for kUpdQ blocks per entry: block b takes rows
+// (b % kUpdQ) kUpdRows .. of entry b / kUpdQ (with swz, of entry
+// xcd_tile(b / kUpdQ, G / kUpdQ)), then strides by G / kUpdQ entries; G is
+// a multiple of kUpdQ.  body(it, row0).
+template <class F>
+__device__ __forceinline__ void for_tile_quarters(const TileList& tl, F&& body, int pin = 0, bool idle = false) {
+    const int b0 = blockIdx.x, G = gridDim.x / kUpdQ, e0 = b0 / kUpdQ, row0 = (b0 % kUpdQ) * kUpdRows;
+    Item it = tl.items[tl.swz ? xcd_tile(e0, G) : e0];
+    const int n = *tl.cnt;
+    use_now(it.inst);
+    use_now(it.tx);
+    use_now(pin);
+    if (idle) return;
+    for (int e = e0; e < n;) {
+        body(it, row0);
+        e += G;
+        if (e < n) it = tl.items[e];  // only lists without swz have more entries than block groups
+    }
+}
+
 // SSP-RK3 buffer roles from the solve's rotation a = I.rot (U^n in U[a]):
 // stage 0 reads U[a] and writes U1 to U[a+1]; stage 1 reads U[a+1] and
 // writes U2 to U[a+2]; stage 2 reads U[a+2] and writes U^{n+1} to U[a+1];
@@ -1203,10 +1230,10 @@ __device__ __forceinline__ void load_around(const Inst& I, const double* __restr
 // around it -- before using any of it, so the global loads overlap.  The
 // arithmetic and its order are src/FLICDdivB.f90:38-54 and
 // src/timeStep.f90:107-118 as written.
-__device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const Item it, int stage) {
+__device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const Item it, int row0, int stage) {
     if (!pass_ready(cw)) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + row0 + (int)(threadIdx.x / kCellT);
     if (i < s || j < s || i > ex + 1 || j > ey + 1) return;
     const size_t idx = (size_t)j * mx + i, fp = (size_t)mx * I.my;
     const bool inner = i <= ex && j <= ey;
@@ -1352,10 +1379,11 @@ __device__ __forceinline__ void update_tile_full(const Inst& I, const Ctl& cw, c
     if (lower) I.ctl[CTL_ANY_LOWER] = 1;
 }
 
+// kUpdQ one-wave blocks per tile (for_tile_quarters)
 template <bool ONE>
-__global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
-                                                int stage, uint32_t* __restrict__ marks,
-                                                uint32_t* __restrict__ amarks, uint32_t stamp, int ninst) {
+__global__ __launch_bounds__(64) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
+                                               int stage, uint32_t* __restrict__ marks,
+                                               uint32_t* __restrict__ amarks, uint32_t stamp, int ninst) {
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
     // Block 0 records, for the launch that ends the iteration, whether the
     // pass is ready (its update and detect run here), whether it ends with
@@ -1381,29 +1409,32 @@ __global__ __launch_bounds__(256) void k_update(const Inst* __restrict__ insts, 
         // tiles are its rows of tiles in order; any grid covers them
         const int nt = one.ntx * ((one.my + kCellT - 1) / kCellT);
         const int lx = threadIdx.x % kCellT, ly = threadIdx.x / kCellT;
-        for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-            const int i = (t % one.ntx) * kCellT + lx, j = (t / one.ntx) * kCellT + ly;
+        for (int v = blockIdx.x; v < nt * kUpdQ; v += gridDim.x) {
+            const int t = v / kUpdQ;
+            const int i = (t % one.ntx) * kCellT + lx, j = (t / one.ntx) * kCellT + (v % kUpdQ) * kUpdRows + ly;
             if (i < one.mx && j < one.my) one.scheme[(size_t)j * one.mx + i] = (uint8_t)one.R;
         }
         return;
     }
-    for_tiles(tl, [&](const Item it) {
+    for_tile_quarters(tl, [&](const Item it, int row0) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
         if (stamp != 0) {
             // a later iteration's active tiles: this is the kernel that
             // visits all of them, so it marks the scan candidates, and, in
             // a ready pass, the tiles whose cells the lowering after detect
-            // replaces
-            if (cw.run && !nomarks) mark_candidates(I, it, marks, stamp);
-            if (threadIdx.x == 0 && pass_ready(cw)) amarks[I.tile_off + it.ty * I.ntx + it.tx] = stamp;
+            // replaces (the tile's first quarter does)
+            if (row0 == 0) {
+                if (cw.run && !nomarks) mark_candidates(I, it, marks, stamp);
+                if (threadIdx.x == 0 && pass_ready(cw)) amarks[I.tile_off + it.ty * I.ntx + it.tx] = stamp;
+            }
         } else {
             // the first pass, every tile: the stage set-up's scheme = R
             // (src/timeStep.f90:79), read first here
-            const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
+            const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + row0 + (int)(threadIdx.x / kCellT);
             if (i < I.mx && j < I.my) I.scheme[(size_t)j * I.mx + i] = (uint8_t)I.R;
         }
-        update_tile(I, cw, it, stage);
+        update_tile(I, cw, it, row0, stage);
     }, c1.run | c1.any_inadm | c1.mode, stamp != 0 && nomarks);
 }
 
@@ -2004,7 +2035,7 @@ void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_
         HMHD_LAUNCH1(k_update_first, g1(grid > 0 ? grid : 1), dim3(256), tl, stage, L.ninst);
         return;
     }
-    HMHD_LAUNCH1(k_update, g1(grid > 0 ? grid : 1), dim3(256), tl, stage, L.marks, L.amarks, stamp, L.ninst);
+    HMHD_LAUNCH1(k_update, g1(kUpdQ * (grid > 0 ? grid : 1)), dim3(64), tl, stage, L.marks, L.amarks, stamp, L.ninst);
 }
 #undef HMHD_LAUNCH1
 
