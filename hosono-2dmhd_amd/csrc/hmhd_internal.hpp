@@ -87,6 +87,9 @@ struct Item {
 // tile sizes: every per-cell kernel works on the same 16 x 16 tiles of the
 // padded grid, so one active-tile list serves a whole MOOD iteration
 constexpr int kCellT = 16;
+// tiles per block of the launch that ends a MOOD iteration (a wave each),
+// and that launch's block size
+constexpr int kScanTiles = 4, kEndIterThreads = 64 * kScanTiles;
 constexpr int kFaceTX = kCellT, kFaceTY = kCellT;
 
 // A list of tiles whose length is a device count: the active list the

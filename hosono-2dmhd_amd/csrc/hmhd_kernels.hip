@@ -1537,7 +1537,6 @@ __device__ __forceinline__ void publish_snapshot(int32_t* __restrict__ snap, int
 //
 // kScanTiles tiles per block, one wave each; the block appends its tiles
 // with one atomic, so the list count sees one update per block
-constexpr int kScanTiles = 4;
 //
 // One more block ends the MOOD iteration (the decisions read only flags
 // earlier kernels set, and no other block reads a word they write: the
@@ -1562,7 +1561,7 @@ __device__ __forceinline__ bool low_near(const Inst& I, int x0, int y0, int lane
 // Four waves per SIMD in both forms (about 100-120 VGPRs); at five the
 // many-solve form spills.
 template <bool ONE>
-__global__ __launch_bounds__(64 * kScanTiles, 1) void k_end_iter(const Inst* __restrict__ insts, const Inst one,
+__global__ __launch_bounds__(kEndIterThreads, 1) void k_end_iter(const Inst* __restrict__ insts, const Inst one,
                                                               const Item* __restrict__ items, int npad,
                                                               Item* __restrict__ act, Item* __restrict__ actf,
                                                               int32_t* __restrict__ act_count,
@@ -1973,7 +1972,7 @@ void launch_end_iter(const LaunchSet& L, int w, uint32_t stamp, int stage, int32
                      hipStream_t st) {
     const int nscan = L.npad > 0 ? (L.npad + kScanTiles - 1) / kScanTiles : 1;
     const int sync_gx = L.sync_gx > 0 ? L.sync_gx : 0;
-    HMHD_LAUNCH1(k_end_iter, g1(sync_gx * L.ninst + (snap ? 1 : 0) + nscan), dim3(64 * kScanTiles), L.pad, L.npad,
+    HMHD_LAUNCH1(k_end_iter, g1(sync_gx * L.ninst + (snap ? 1 : 0) + nscan), dim3(kEndIterThreads), L.pad, L.npad,
                  L.act[w], L.actf[w], L.act_count + 2 * w, L.marks, L.amarks, stamp, L.ninst, stage, L.ctl, w, snap,
                  seq, sync_gx);
 }

@@ -452,7 +452,7 @@ int Engine::commit() {
     L.pad = d_pad_;
     L.npad = (int)pad.size();
     L.ninst = (int)n;
-    L.sync_gx = std::min((max_sync_units + 255) / 256, 256);
+    L.sync_gx = std::min((max_sync_units + kEndIterThreads - 1) / kEndIterThreads, 65536 / kEndIterThreads);
     L.cell = d_cell_;
     L.ncell = (int)cell.size();
     L.act[0] = d_act_;
