@@ -105,6 +105,19 @@ def test_steps_bitwise(name, steps):
         _compare_state(orc, gpu, f"{name} step {n + 1}")
 
 
+@pytest.mark.parametrize("name,n,R,steps", [
+    ("blast", 48, 3, 10),  # R = 3 (eight lanes a face on pairs), fallback through radii 2, 1 and 0
+    ("ot", 48, 1, 12),     # R = 1 (four lanes a face), periodic shocks
+])
+def test_steps_bitwise_other_radii(name, n, R, steps):
+    """Later MOOD passes at every lower radius of R = 3 and R = 1 solves, on
+    the later-pass face kernel's lane pairs and the four-block tile update."""
+    cfg, g, orc, gpu = _pair(name, n=n, R=R)
+    for k in range(steps):
+        _step_both(orc, gpu)
+        _compare_state(orc, gpu, f"{name} {n}^2 R={R} step {k + 1}")
+
+
 @pytest.mark.parametrize("name,n,R", [
     ("ot", 6, 1), ("ot", 8, 2), ("ot", 10, 3),  # N = 2 NGC: guard and image bands meet
     ("blast", 6, 1), ("blast", 8, 2),           # one tile holds the whole padded grid
