@@ -1308,16 +1308,15 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = rk_combine(stage, ub[v], un[v]);
 }
 
-// The stage's first update in a ready first pass: every cell is in rc and
-// rcF (they are set everywhere until the first update) and its scheme is R.
-// The arithmetic is update_tile's; the loads come in two phases -- the
+// The stage's first update of cell (i, j) in a ready first pass: every cell
+// is in rc and rcF (they are set everywhere until the first update) and its
+// scheme is R.  The arithmetic is update_tile's; the loads come in two phases -- the
 // FLICD stencil, the fluxes and U for the FV update and the RK combination,
 // then U around the cell for detection -- so that the kernel needs fewer
 // registers and more waves hide the loads of a full grid.
-__device__ __forceinline__ void update_tile_full(const Inst& I, const Ctl& cw, const Item it, int stage) {
+__device__ __forceinline__ void update_cell_full(const Inst& I, const Ctl& cw, int i, int j, int stage) {
     if (!pass_ready(cw)) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i < s || j < s || i > ex + 1 || j > ey + 1) return;
     const size_t idx = (size_t)j * mx + i, fp = (size_t)mx * I.my;
     const bool inner = i <= ex && j <= ey;
@@ -1440,7 +1439,11 @@ __global__ __launch_bounds__(64) void k_update(const Inst* __restrict__ insts, c
 
 // The stage's first update (iteration 0, every tile of every solve): the
 // control words of k_update's block 0, the scheme reset to R, and, in a
-// ready pass, update_tile_full.  At most 168 VGPRs: three waves per SIMD.
+// ready pass, update_cell_full.  At most 168 VGPRs: three waves per SIMD.
+// a single solve's full update: regions of kFullRX x kFullRY cells a block
+constexpr int kFullRX = 64, kFullRY = 4;
+static_assert(kFullRX * kFullRY == 256, "a cell per thread");
+
 template <bool ONE>
 __global__ __launch_bounds__(256, 3) void k_update_first(const Inst* __restrict__ insts, const Inst one,
                                                          const TileList tl, int stage, int ninst) {
@@ -1465,13 +1468,29 @@ __global__ __launch_bounds__(256, 3) void k_update_first(const Inst* __restrict_
         }
         return;
     }
+    if constexpr (ONE) {
+        // a single solve: kFullRX x kFullRY-cell regions in row-major order
+        // (xcd_tile's bands when the grid has one block per region), so a
+        // wave's load of a plane is one 512-byte run of a row
+        const int nrx = (one.mx + kFullRX - 1) / kFullRX, nreg = nrx * ((one.my + kFullRY - 1) / kFullRY);
+        const int lx = threadIdx.x % kFullRX, ly = threadIdx.x / kFullRX;
+        const bool one_each = (int)gridDim.x == nreg;
+        for (int b = blockIdx.x; b < nreg; b += gridDim.x) {
+            const int r = one_each ? xcd_tile(b, nreg) : b;
+            const int i = (r % nrx) * kFullRX + lx, j = (r / nrx) * kFullRY + ly;
+            // the stage set-up's scheme = R (src/timeStep.f90:79), read first here
+            if (i < one.mx && j < one.my) one.scheme[(size_t)j * one.mx + i] = (uint8_t)one.R;
+            update_cell_full(one, c1, i, j, stage);
+        }
+        return;
+    }
     for_tiles(tl, [&](const Item it) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
         // the stage set-up's scheme = R (src/timeStep.f90:79), read first here
         const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
         if (i < I.mx && j < I.my) I.scheme[(size_t)j * I.mx + i] = (uint8_t)I.R;
-        update_tile_full(I, cw, it, stage);
+        update_cell_full(I, cw, i, j, stage);
     }, c1.run | c1.any_inadm | c1.mode, false);
 }
 
@@ -2031,6 +2050,10 @@ void launch_face_full(const LaunchSet& L, int R, int v, const Item* items, int n
 void launch_update(const LaunchSet& L, TileList tl, int grid, int stage, uint32_t stamp, hipStream_t st) {
     // block 0 always runs: it writes the control words the iteration's end reads
     if (stamp == 0) {
+        // a single solve's full update strides over its regions (one block
+        // each when the grid asks for every tile)
+        if (L.one && grid >= L.npad)
+            grid = ((L.inst1.mx + kFullRX - 1) / kFullRX) * ((L.inst1.my + kFullRY - 1) / kFullRY);
         HMHD_LAUNCH1(k_update_first, g1(grid > 0 ? grid : 1), dim3(256), tl, stage, L.ninst);
         return;
     }
