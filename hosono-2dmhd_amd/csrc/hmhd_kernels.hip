@@ -1439,7 +1439,7 @@ __global__ __launch_bounds__(64) void k_update(const Inst* __restrict__ insts, c
 
 // The stage's first update (iteration 0, every tile of every solve): the
 // control words of k_update's block 0, the scheme reset to R, and, in a
-// ready pass, update_cell_full.  At most 168 VGPRs: three waves per SIMD.
+// ready pass, update_cell_full.  At most 168 VGPRs (159 as built): three waves per SIMD.
 // a single solve's full update: regions of kFullRX x kFullRY cells a block
 constexpr int kFullRX = 64, kFullRY = 4;
 static_assert(kFullRX * kFullRY == 256, "a cell per thread");
