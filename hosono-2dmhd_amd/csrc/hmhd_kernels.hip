@@ -1230,10 +1230,9 @@ __device__ __forceinline__ void load_around(const Inst& I, const double* __restr
 // around it -- before using any of it, so the global loads overlap.  The
 // arithmetic and its order are src/FLICDdivB.f90:38-54 and
 // src/timeStep.f90:107-118 as written.
-__device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const Item it, int row0, int stage) {
+__device__ __forceinline__ void update_cell(const Inst& I, const Ctl& cw, int i, int j, int stage) {
     if (!pass_ready(cw)) return;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + row0 + (int)(threadIdx.x / kCellT);
     if (i < s || j < s || i > ex + 1 || j > ey + 1) return;
     const size_t idx = (size_t)j * mx + i, fp = (size_t)mx * I.my;
     const bool inner = i <= ex && j <= ey;
@@ -1306,6 +1305,12 @@ __device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const 
     double* __restrict__ Uout = ubuf(I, sb.out);
 #pragma unroll
     for (int v = 0; v < NCONS; v++) Uout[v * plane + ci] = rk_combine(stage, ub[v], un[v]);
+}
+
+// update_cell for row row0 + lane / kCellT, column lane % kCellT of tile it
+__device__ __forceinline__ void update_tile(const Inst& I, const Ctl& cw, const Item it, int row0, int stage) {
+    update_cell(I, cw, it.tx * kCellT + (int)(threadIdx.x % kCellT), it.ty * kCellT + row0 + (int)(threadIdx.x / kCellT),
+                stage);
 }
 
 // The stage's first update of cell (i, j) in a ready first pass: every cell
@@ -1414,6 +1419,25 @@ __global__ __launch_bounds__(64) void k_update(const Inst* __restrict__ insts, c
             if (i < one.mx && j < one.my) one.scheme[(size_t)j * one.mx + i] = (uint8_t)one.R;
         }
         return;
+    }
+    if constexpr (ONE) {
+        // Every tile of a single solve listed (the stage's full update when
+        // its first pass was not ready): the tiles' marks, then the update on
+        // runs of 64 cells of a row, a block each, so a wave's load of a plane
+        // is one 512-byte run
+        const int nt = one.ntx * ((one.my + kCellT - 1) / kCellT);
+        if (stamp != 0 && *tl.cnt == nt) {
+            if (nomarks) return;  // nothing to mark or update (for_tile_quarters' idle launch)
+            for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+                const Item it{0, (int16_t)(t % one.ntx), (int16_t)(t / one.ntx)};
+                if (c1.run && !nomarks) mark_candidates(one, it, marks, stamp);
+                if (threadIdx.x == 0 && pass_ready(c1)) amarks[one.tile_off + t] = stamp;
+            }
+            const int nrx = (one.mx + 63) / 64, nseg = nrx * one.my;
+            for (int g = blockIdx.x; g < nseg; g += gridDim.x)
+                update_cell(one, c1, (g % nrx) * 64 + (int)threadIdx.x, g / nrx, stage);
+            return;
+        }
     }
     for_tile_quarters(tl, [&](const Item it, int row0) {
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
