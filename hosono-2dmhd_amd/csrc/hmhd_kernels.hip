@@ -17,8 +17,8 @@
 //              iteration needs, the next inner-loop round's lowering and the
 //              loop decisions
 //
-// The first pass of a stage runs k_face in FULL mode: every face at the top
-// radius, admissibility checked on the way.  Every later iteration (a new
+// The first pass of a stage runs k_face_full: every face at the top radius,
+// admissibility checked on the way.  Every later iteration (a new
 // MOOD pass, or another round of the inner admissibility loop when a
 // reconstructed state was inadmissible) runs k_face on the active tiles
 // only (the inner round's lowering ran in the scan launch before it): of
