@@ -73,8 +73,11 @@ struct Inst {
     uint8_t* faceRR;  // [4][my][mx]
     uint8_t* fluxRR;  // [2][my][mx] radius of the last flux of the x / y face
     // per 16 x 16 tile ([ty][ntx]): where the inadmissible face sides the
-    // face kernel's last visit of the tile left lie (bad_bits)
+    // face kernels' last visit of the tile left lie (bad_bits); the later
+    // passes' face kernel takes a tile in two blocks, one per half (rows
+    // 0-7, 8-15), each with its word (tbad, tbadh), and a reader ORs the two
     uint32_t* tbad;
+    uint32_t* tbadh;
     int32_t* ctl;
 };
 

@@ -215,13 +215,12 @@ __device__ __forceinline__ void for_tiles(const TileList& tl, F&& body, int pin 
 constexpr int kUpdRows = 4, kUpdQ = kCellT / kUpdRows;
 static_assert(kUpdRows * kCellT == 64, "one wave per quarter tile");
 
-// for_tiles for kUpdQ blocks per entry: block b takes rows
-// (b % kUpdQ) kUpdRows .. of entry b / kUpdQ (with swz, of entry
-// xcd_tile(b / kUpdQ, G / kUpdQ)), then strides by G / kUpdQ entries; G is
-// a multiple of kUpdQ.  body(it, row0).
-template <class F>
-__device__ __forceinline__ void for_tile_quarters(const TileList& tl, F&& body, int pin = 0, bool idle = false) {
-    const int b0 = blockIdx.x, G = gridDim.x / kUpdQ, e0 = b0 / kUpdQ, row0 = (b0 % kUpdQ) * kUpdRows;
+// for_tiles for P blocks per entry: block b takes part b % P of entry b / P
+// (with swz, of entry xcd_tile(b / P, G / P)), then strides by G / P
+// entries; G is a multiple of P.  body(it, part).
+template <int P, class F>
+__device__ __forceinline__ void for_tile_parts(const TileList& tl, F&& body, int pin = 0, bool idle = false) {
+    const int b0 = blockIdx.x, G = gridDim.x / P, e0 = b0 / P, part = b0 % P;
     Item it = tl.items[tl.swz ? xcd_tile(e0, G) : e0];
     const int n = *tl.cnt;
     use_now(it.inst);
@@ -229,7 +228,7 @@ __device__ __forceinline__ void for_tile_quarters(const TileList& tl, F&& body, 
     use_now(pin);
     if (idle) return;
     for (int e = e0; e < n;) {
-        body(it, row0);
+        body(it, part);
         e += G;
         if (e < n) it = tl.items[e];  // only lists without swz have more entries than block groups
     }
@@ -283,7 +282,10 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
     const Item it = items[xcd_tile(blockIdx.x, gridDim.x)];
     const Inst& I = insts[it.inst];
     // the first pass's face kernel ORs into the tile's bad_bits word
-    if (threadIdx.x == 0) I.tbad[it.ty * I.ntx + it.tx] = 0;
+    if (threadIdx.x == 0) {
+        I.tbad[it.ty * I.ntx + it.tx] = 0;
+        I.tbadh[it.ty * I.ntx + it.tx] = 0;
+    }
     const int i = it.tx * kCellT + (threadIdx.x % kCellT), j = it.ty * kCellT + (threadIdx.x / kCellT);
     if (i >= I.mx || j >= I.my) return;
     const size_t idx = (size_t)j * I.mx + i, plane = (size_t)I.mx * I.my;
@@ -309,15 +311,17 @@ __global__ __launch_bounds__(256) void k_stage_init(const Inst* __restrict__ ins
 // Later passes (k_face, face_tile_iter): one iteration of the inner
 // admissibility loop fused with the flux recomputation, over a compacted
 // list of the faces whose inputs changed, on the 16 x 16 tiles every
-// per-cell kernel uses (blocks stride over the tile list); at R <= 2 the
-// block has 512 threads so a sparse pass costs one round of lanes.
+// per-cell kernel uses, a tile as two blocks of eight rows each (blocks
+// stride over the tile list), so a tile's loads are in flight on two CUs;
+// at R <= 2 the block has 256 threads, one per face of its half.
 // Inadmissibility is recorded per face side in four byte planes, so each
 // byte has a single writer.  Register budget: two waves per SIMD at R <= 2;
 // R = 3 keeps one (its stencils would spill at two).
-__host__ __device__ constexpr int face_threads(int R, bool full) { return (R >= 3 || full ? 1 : 2) * kFaceTX * kFaceTY; }
+constexpr int kFaceTYH = kFaceTY / 2;  // rows of a tile per later-pass face block
+__host__ __device__ constexpr int face_threads(int R) { return (R >= 3 ? 1 : 2) * kFaceTX * kFaceTYH; }
 template <int R>
 constexpr int face_lds_cells() {
-    return (kFaceTX + 2 * R + 1) * (kFaceTY + 2 * R + 1);
+    return (kFaceTX + 2 * R + 1) * (kFaceTYH + 2 * R + 1);
 }
 
 // Where in a tile the cell of an inadmissible face side lies, for the scan
@@ -722,20 +726,21 @@ __device__ __forceinline__ void mark_candidates(const Inst& I, const Item it, ui
 // the single-solve kernel, latency-bound on sparse tiles, keeps lists by
 // direction only.
 template <int R, bool BYR>
-__device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int stage,
+__device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, const Item it, int hh, int stage,
                                                double* __restrict__ sU, uint32_t* __restrict__ lst,
                                                int* __restrict__ lcnt, double* __restrict__ sPV,
                                                double* __restrict__ sQW) {
-    constexpr int TX = kFaceTX, TY = kFaceTY, NC = TX * TY, H = R + 1, NT = face_threads(R, false);
+    // half hh of the tile: its rows hh TY .. hh TY + TY - 1
+    constexpr int TX = kFaceTX, TY = kFaceTYH, NC = TX * TY, H = R + 1, NT = face_threads(R);
     constexpr int LX = TX + 2 * R + 1, LP = face_lds_cells<R>();
 
     if (I.R != R || !cw.run) return;
     const bool inner = cw.mode == MODE_INNER;
     const int s = I.ngc, ex = I.ngc + I.nx - 1, ey = I.ngc + I.ny - 1, mx = I.mx;
-    const int i0 = it.tx * TX, j0 = it.ty * TY;
+    const int i0 = it.tx * TX, j0 = it.ty * kFaceTY + hh * TY, ly0 = hh * TY;  // ly0: the half's first tile row
     const size_t fplane = (size_t)mx * I.my;
     const uint8_t* flag = cw.mode == MODE_NEWPASS ? I.rc : I.rec;
-    uint32_t* const tbad = I.tbad + it.ty * I.ntx + it.tx;
+    uint32_t* const tbad = (hh ? I.tbadh : I.tbad) + it.ty * I.ntx + it.tx;
     // the faces to rebuild, in lists by direction and (BYR) radius: bucket
     // d K + rr of lst (K = R + 1 with BYR, else 1 and rr = 0; NC entries
     // each), counts in lcnt
@@ -744,14 +749,15 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     if (threadIdx.x == 0) *tbad = 0;  // before the barrier, so the later ORs land on it
     __syncthreads();
 
-    // With one face per thread and a U tile of at most one cell per thread,
-    // each thread's cell of the tile is loaded together with its face's
+    // With one face per thread and a U tile of at most two cells per thread,
+    // each thread's cells of the tile are loaded together with its face's
     // classification values, and written to LDS only if the tile has faces
     // to rebuild: the rebuild waits on no further round trip.
-    constexpr bool kPre = NT == 2 * NC && LP <= NT;
+    constexpr int KP = (LP + NT - 1) / NT;
+    constexpr bool kPre = NT == 2 * NC && KP <= 2;
     const double* __restrict__ Uin = ubuf(I, stage_bufs(I.rot, stage).in);
     const size_t uplane = (size_t)I.nx * I.ny;
-    double pre[NCONS];
+    double pre[KP][NCONS];
     uint32_t bits = 0;
     // The solve's GP weights for radii 1..R and its quadrature weights, for
     // LDS: a lane's weights depend on its face's radius and Gauss point, so
@@ -771,11 +777,14 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const int i = i0 + c % TX, j = j0 + c / TX;
         const bool dom = i >= s - 2 && i <= ex + 3 && j >= s - 2 && j <= ey + 3;
         if constexpr (kPre) {
-            const int u = f < LP ? f : LP - 1, uy = u / LX, ux = u - uy * LX;
-            const int kxx = map_axis(i0 - H + ux, s, I.nx, I.bc), kyy = map_axis(j0 - H + uy, s, I.ny, I.bc);
-            const double* src = Uin + (size_t)kyy * I.nx + kxx;
 #pragma unroll
-            for (int v = 0; v < NCONS; v++) pre[v] = src[v * uplane];
+            for (int p = 0; p < KP; p++) {
+                const int u0 = f + p * NT, u = u0 < LP ? u0 : LP - 1, uy = u / LX, ux = u - uy * LX;
+                const int kxx = map_axis(i0 - H + ux, s, I.nx, I.bc), kyy = map_axis(j0 - H + uy, s, I.ny, I.bc);
+                const double* src = Uin + (size_t)kyy * I.nx + kxx;
+#pragma unroll
+                for (int v = 0; v < NCONS; v++) pre[p][v] = src[v * uplane];
+            }
         }
         // every value the classification reads, loaded at once from clamped
         // indices (a cell off the range reads a valid cell and is dropped)
@@ -807,8 +816,8 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         const bool keepF = !fl || rrF == rr_now;
         if (keepL && keepR && keepF) {
             if (fl) I.FLB[d * fplane + idx] = raw;
-            if (recL && badL) bits |= bad_bits(d == 0 ? c % TX - 1 : c % TX, d == 0 ? c / TX : c / TX - 1);
-            if (recR && badR) bits |= bad_bits(c % TX, c / TX);
+            if (recL && badL) bits |= bad_bits(d == 0 ? c % TX - 1 : c % TX, ly0 + (d == 0 ? c / TX : c / TX - 1));
+            if (recR && badR) bits |= bad_bits(c % TX, ly0 + c / TX);
         } else {
             const int b = d * K + (BYR ? rr_now : 0), slot = atomicAdd(&lcnt[b], 1);
             lst[b * NC + slot] = (uint32_t)c | (recL ? 0x100u : 0u) | (recR ? 0x200u : 0u) | (fl ? 0x400u : 0u) |
@@ -860,9 +869,13 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
     if (nwaves > 0) {
         // stage the U tile [i0-H, i0+TX-1+R] x [j0-H, j0+TY-1+R] in LDS
         if constexpr (kPre) {
-            if (threadIdx.x < LP) {
 #pragma unroll
-                for (int v = 0; v < NCONS; v++) sU[v * LP + threadIdx.x] = pre[v];
+            for (int p = 0; p < KP; p++) {
+                const int u = (int)threadIdx.x + p * NT;
+                if (u < LP) {
+#pragma unroll
+                    for (int v = 0; v < NCONS; v++) sU[v * LP + u] = pre[p][v];
+                }
             }
         } else {
             const int oi = i0 - H, oj = j0 - H;
@@ -914,39 +927,40 @@ __device__ __forceinline__ void face_tile_iter(const Inst& I, const Ctl& cw, con
         if constexpr (BYR) {
             if (isx)
                 face_group<R, 0>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - 1}, aR, idx, idx - 1, recL, recR, fl, rr_now,
-                                rrL, rrR, r, lane - r, n, lx, ly, bits);
+                                rrL, rrR, r, lane - r, n, lx, ly0 + ly, bits);
             else
                 face_group<R, 1>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - LX}, aR, idx, idx - mx, recL, recR, fl, rr_now,
-                                rrL, rrR, r, lane - r, n, lx, ly, bits);
+                                rrL, rrR, r, lane - r, n, lx, ly0 + ly, bits);
         } else {
             // lane r of the group: point r / 2, side r % 2 (1: R, the cell itself)
             const int q = r >> 1, sd = r & 1;
             if (isx)
                 face_group_pair<R, 0>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - (sd ? 0 : 1)}, idx, idx - 1, recL, recR,
-                                      fl, rr_now, rrL, rrR, q, sd, lane - r, lx, ly, bits);
+                                      fl, rr_now, rrL, rrR, q, sd, lane - r, lx, ly0 + ly, bits);
             else
                 face_group_pair<R, 1>(I, ph, sPV, sQW, LdsAcc<LX, LP>{sU + cc - (sd ? 0 : LX)}, idx, idx - mx, recL,
-                                      recR, fl, rr_now, rrL, rrR, q, sd, lane - r, lx, ly, bits);
+                                      recR, fl, rr_now, rrL, rrR, q, sd, lane - r, lx, ly0 + ly, bits);
         }
     }
     report_bad(I, tbad, bits);
 }
 
 template <int R, bool ONE>
-__global__ __launch_bounds__(face_threads(R, false), R >= 3 ? 1 : 2) void k_face(const Inst* __restrict__ insts,
+__global__ __launch_bounds__(face_threads(R), R >= 3 ? 1 : 2) void k_face(const Inst* __restrict__ insts,
                                                                                  const Inst one, const TileList tl,
                                                                                  int stage) {
     __shared__ double sU[NCONS * face_lds_cells<R>()];
-    __shared__ uint32_t lst[2 * (ONE ? 1 : gp_nq(R)) * kFaceTX * kFaceTY];
+    __shared__ uint32_t lst[2 * (ONE ? 1 : gp_nq(R)) * kFaceTX * kFaceTYH];
     __shared__ int lcnt[2 * (ONE ? 1 : gp_nq(R))];
     __shared__ double sPV[gp_pv_off(R + 1)], sQW[(R + 1) * 4];
     const Ctl c1 = ONE ? ctl_of(one.ctl) : Ctl{};
     bool later = false;
-    for_tiles(tl, [&](const Item it) {
+    // two blocks per tile, one per half (for_tile_parts)
+    for_tile_parts<2>(tl, [&](const Item it, int hh) {
         if (later) __syncthreads();  // the previous tile's LDS reads are done
         later = true;
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
-        face_tile_iter<R, !ONE>(I, ONE ? c1 : ctl_of(I.ctl), it, stage, sU, lst, lcnt, sPV, sQW);
+        face_tile_iter<R, !ONE>(I, ONE ? c1 : ctl_of(I.ctl), it, hh, stage, sU, lst, lcnt, sPV, sQW);
     }, c1.run | c1.mode);
 }
 
@@ -1383,7 +1397,7 @@ __device__ __forceinline__ void update_cell_full(const Inst& I, const Ctl& cw, i
     if (lower) I.ctl[CTL_ANY_LOWER] = 1;
 }
 
-// kUpdQ one-wave blocks per tile (for_tile_quarters)
+// kUpdQ one-wave blocks per tile (for_tile_parts)
 template <bool ONE>
 __global__ __launch_bounds__(64) void k_update(const Inst* __restrict__ insts, const Inst one, const TileList tl,
                                                int stage, uint32_t* __restrict__ marks,
@@ -1427,7 +1441,7 @@ __global__ __launch_bounds__(64) void k_update(const Inst* __restrict__ insts, c
         // is one 512-byte run
         const int nt = one.ntx * ((one.my + kCellT - 1) / kCellT);
         if (stamp != 0 && *tl.cnt == nt) {
-            if (nomarks) return;  // nothing to mark or update (for_tile_quarters' idle launch)
+            if (nomarks) return;  // nothing to mark or update (for_tile_parts' idle launch)
             for (int t = blockIdx.x; t < nt; t += gridDim.x) {
                 const Item it{0, (int16_t)(t % one.ntx), (int16_t)(t / one.ntx)};
                 if (c1.run && !nomarks) mark_candidates(one, it, marks, stamp);
@@ -1439,7 +1453,8 @@ __global__ __launch_bounds__(64) void k_update(const Inst* __restrict__ insts, c
             return;
         }
     }
-    for_tile_quarters(tl, [&](const Item it, int row0) {
+    for_tile_parts<kUpdQ>(tl, [&](const Item it, int q) {
+        const int row0 = q * kUpdRows;
         const Inst& I = inst_at<ONE>(insts, one, it.inst);
         const Ctl cw = ONE ? c1 : ctl_of(I.ctl);
         if (stamp != 0) {
@@ -1677,7 +1692,8 @@ __global__ __launch_bounds__(kEndIterThreads, 1) void k_end_iter(const Inst* __r
     if (dolow) ld.load(I, it, lane);
     const int nty = (I.my + kCellT - 1) / kCellT, ta = it.tx + lane % 3 - 1, tb = it.ty + lane / 3 - 1;
     const bool tin = lane < 9 && ta >= 0 && tb >= 0 && ta < I.ntx && tb < nty;
-    const uint32_t tf = scanned ? I.tbad[(tin ? tb : it.ty) * I.ntx + (tin ? ta : it.tx)] : 0u;
+    const int tw = (tin ? tb : it.ty) * I.ntx + (tin ? ta : it.tx);
+    const uint32_t tf = scanned ? I.tbad[tw] | I.tbadh[tw] : 0u;
     if (dolow) {
         // the window to this wave's LDS rows (a wave's LDS accesses complete
         // in order), then the lowering
@@ -2022,9 +2038,9 @@ void launch_end_iter(const LaunchSet& L, int w, uint32_t stamp, int stage, int32
 
 void launch_face_iter(const LaunchSet& L, TileList tl, int grid, int R, int stage, hipStream_t st) {
     if (grid <= 0) return;
-    const dim3 g = g1(grid);
+    const dim3 g = g1(2 * grid);  // two blocks per tile, one per half
 #define HMHD_FACE(RR, ONE)                                                                                     \
-    hipLaunchKernelGGL((k_face<RR, ONE>), g, dim3(face_threads(RR, false)), 0, st, L.insts, L.inst1, tl, stage);
+    hipLaunchKernelGGL((k_face<RR, ONE>), g, dim3(face_threads(RR)), 0, st, L.insts, L.inst1, tl, stage);
 #define HMHD_FACE_R(ONE)   \
     if (R == 1) {          \
         HMHD_FACE(1, ONE)  \

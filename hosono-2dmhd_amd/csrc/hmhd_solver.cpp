@@ -401,10 +401,14 @@ int Engine::commit() {
         h.dev = d;
         table[k] = d;
     }
-    if ((rc = grow(&d_tbad_, cap_tbad_, ntiles + 1, false))) return rc;
-    if ((rc = hip_check(hipMemsetAsync(d_tbad_, 0, sizeof(uint32_t) * (ntiles + 1), st_), "hipMemsetAsync(tbad)")))
+    if ((rc = grow(&d_tbad_, cap_tbad_, 2 * (ntiles + 1), false))) return rc;
+    if ((rc = hip_check(hipMemsetAsync(d_tbad_, 0, sizeof(uint32_t) * 2 * (ntiles + 1), st_),
+                        "hipMemsetAsync(tbad)")))
         return rc;
-    for (size_t k = 0; k < n; k++) inst_[k].dev.tbad = table[k].tbad = d_tbad_ + table[k].tile_off;
+    for (size_t k = 0; k < n; k++) {
+        inst_[k].dev.tbad = table[k].tbad = d_tbad_ + table[k].tile_off;
+        inst_[k].dev.tbadh = table[k].tbadh = d_tbad_ + (ntiles + 1) + table[k].tile_off;
+    }
     if (n && (rc = hip_check(hipMemcpyAsync(d_insts_, table.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, st_),
                              "hipMemcpyAsync(table)")))
         return rc;
